@@ -1,0 +1,102 @@
+/* qcd_hip.h -- device-resident extension API of the MI355X lattice-QCD library.
+ *
+ * The drop-in entry points (qcd_wilson.h, qcd_smearing.h, qcd_blas.h) take
+ * host structs and copy data in and out on every call, as the reference's
+ * signatures require.  This header is the fast path: opaque handles whose data
+ * stay in HBM across calls (gauge fields uploaded once, solver state resident).
+ * It has no counterpart in the reference; every entry point below names the
+ * reference routine whose semantics it implements.
+ *
+ * Conventions: functions return 0 on success and non-zero on failure, with a
+ * message available from qcd_hip_last_error().  Host arrays use the
+ * reference's AoS layouts (include/qcd.h:90-126) as interleaved doubles.
+ * All work is ordered on one library stream per process; calls are not
+ * thread-safe.  One GPU, one rank.
+ */
+#ifndef QCD_HIP_H
+#define QCD_HIP_H 1
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qcd_hip_lattice_s *qcd_hip_lattice;
+typedef struct qcd_hip_field_s *qcd_hip_field;
+
+enum { QCD_HIP_F64 = 0, QCD_HIP_F32 = 1, QCD_HIP_F16 = 2 };
+enum { QCD_HIP_SPINOR = 0, QCD_HIP_GAUGE = 1 };
+/* operator selector: D (lib/src/qcd_wilson.c:114), Q = i gamma5 D (:795) */
+enum { QCD_HIP_OP_D = 0, QCD_HIP_OP_Q = 1 };
+
+/* ------------------------------------------------------------ runtime */
+const char *qcd_hip_last_error(void);
+int    qcd_hip_init(int device);
+int    qcd_hip_device_name(char *buf, int len);
+int    qcd_hip_synchronize(void);
+size_t qcd_hip_bytes_in_use(void);
+
+/* ------------------------------------------------------------ lattice
+ * L = {T, X, Y, Z}; theta are the fermion boundary phases
+ * psi(x + L_mu) = e^{i theta_mu} psi(x), applied as e^{+-i theta_mu/L_mu} on
+ * every hop in direction mu (lib/src/qcd_wilson.c:172-186). */
+int  qcd_hip_lattice_create(qcd_hip_lattice *lat, const int L[4], const double theta[4]);
+void qcd_hip_lattice_destroy(qcd_hip_lattice lat);
+long qcd_hip_lattice_volume(qcd_hip_lattice lat);
+
+/* ------------------------------------------------------------ fields */
+int  qcd_hip_spinor_create(qcd_hip_field *f, qcd_hip_lattice lat, int prec, int nrhs);
+int  qcd_hip_gauge_create(qcd_hip_field *f, qcd_hip_lattice lat, int prec);
+void qcd_hip_field_destroy(qcd_hip_field f);
+/* spinor rhs <-> host double[V][4][3][2] (qcd_vector.D) */
+int  qcd_hip_spinor_upload(qcd_hip_field f, int rhs, const double *aos);
+int  qcd_hip_spinor_download(qcd_hip_field f, int rhs, double *aos);
+/* gauge <-> host double[V][4][3][3][2] (qcd_gaugeField.D) */
+int  qcd_hip_gauge_upload(qcd_hip_field f, const double *aos);
+int  qcd_hip_gauge_download(qcd_hip_field f, double *aos);
+/* copy with precision change; same lattice, same kind (and nrhs for spinors) */
+int  qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src);
+
+/* Synthetic inputs, host side (identical bits on every machine with the same
+ * libm): U = reunitarize(exp(i eps H)) per link, H a traceless Hermitian
+ * matrix from seeded N(0,1) draws; spinor entries i.i.d. N(0,1). */
+int qcd_hip_gauge_generate_host(double *aos, long V, double eps, uint64_t seed);
+int qcd_hip_spinor_gaussian_host(double *aos, long V, uint64_t seed);
+
+/* ------------------------------------------------------------ operators
+ * out = D in (op = QCD_HIP_OP_D) or Q in = i gamma5 D in (QCD_HIP_OP_Q),
+ * with m the mass parameter (m + 4 = 1/(2 kappa)) and tm the twisted mass;
+ * every rhs of in is applied.  out must not alias in.
+ * (lib/src/qcd_wilson.c:114-336, :570-792, :795-814) */
+int qcd_hip_dslash(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge,
+                   double m, double tm, int op);
+/* Times `iters` back-to-back applications (after `warmup`) with HIP events
+ * on the library stream; returns the mean per application in *avg_ms. */
+int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge,
+                        double m, double tm, int op, int warmup, int iters, double *avg_ms);
+
+/* ------------------------------------------------------------ smearing
+ * niter iterations of 3D Gaussian smearing of every rhs of v on time-slice t
+ * (t < 0: every time-slice), in place (lib/src/qcd_smearing.c:25-199).
+ * v and gauge must have the same precision. */
+int qcd_hip_gauss_smear(qcd_hip_field v, qcd_hip_field gauge, double alpha, int t, int niter);
+
+/* ------------------------------------------------------------ BLAS (FP64
+ * accumulation, deterministic reduction order for a given lattice size) */
+int qcd_hip_norm2(qcd_hip_field v, int rhs, double *out);
+int qcd_hip_dot(qcd_hip_field a, int ra, qcd_hip_field b, int rb, double out[2]);
+/* y[ry] += a x[rx] (lib/src/qcd_blas.c:414-432; y may alias x) */
+int qcd_hip_axpy(qcd_hip_field y, int ry, double are, double aim, qcd_hip_field x, int rx);
+/* out[ro] = a x[rx] + b y[ry] on all sites, or on time-slice t0 only when
+ * t0 >= 0 (lib/src/qcd_blas.c:237-434: scale, scale3d, add, sub, add3d,
+ * mulVectorC are special cases); y may be NULL for b = 0; out may alias x, y */
+int qcd_hip_lincomb(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx,
+                    double bre, double bim, qcd_hip_field y, int ry, int t0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

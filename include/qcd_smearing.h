@@ -1,0 +1,29 @@
+/* qcd_smearing.h -- 3D Gaussian (Wuppertal) smearing of the MI355X library.
+ *
+ * Drop-in counterpart of the reference's include/qcd_smearing.h:11-40 and
+ * lib/src/qcd_smearing.c:25-199.  One iteration on time-slice t is
+ *    v(x) <- ( v(x) + alpha * sum_{nu=1..3} [ U_nu(x) v(x+nu) + U_nu(x-nu)^+ v(x-nu) ] ) / (1 + 6 alpha)
+ * with no spin structure and no boundary phase; other time-slices are left
+ * unchanged.  The "All" variant updates every time-slice.  Both run on the GPU.
+ */
+#ifndef H_QCD_SMEARING
+#define H_QCD_SMEARING 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* t += a + b over a 24-real spinor (qcd_smearing.h:11-36) */
+#define qcd_SUM_UP_HOPP(t,a,b) \
+   do { int qcd_j_; for (qcd_j_ = 0; qcd_j_ < 24; qcd_j_++) (t)[qcd_j_] += (a)[qcd_j_] + (b)[qcd_j_]; } while (0)
+
+int qcd_gaussIteration3d(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, qcd_uint_4 t);
+/* gaugeCom selects whether the reference exchanges gauge faces; with one rank
+ * it has no effect. */
+int qcd_gaussIteration3dAll(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, qcd_uint_2 gaugeCom);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,706 @@
+/* qcd_oracle.c -- CPU oracle: plain-C restatement of the reference hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see qcd_oracle.h).  Only tests/, the smoke check
+ * and the cpu_baseline leg of bench.py load this; the product never does.
+ *
+ * Each routine names the reference lines it restates.  The reference itself
+ * cannot be compiled in this environment (it needs mpi.h), so this oracle is
+ * pinned by closed forms, identities and brute force in tests/.
+ */
+#include "qcd_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+struct orc_geo {
+   int L[4];
+   long V;
+   long *plus;    /* plus[4*l + mu]  */
+   long *minus;   /* minus[4*l + mu] */
+   double theta[4];
+};
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx cmul(cplx a, cplx b) { cplx r = {a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re}; return r; }
+static inline cplx cadd(cplx a, cplx b) { cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cplx conjc(cplx a) { cplx r = {a.re, -a.im}; return r; }
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+   if (n > 0) omp_set_num_threads(n);
+#else
+   (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------ geometry
+ * lib/src/qcd_init.c:110-124: plus/minus neighbour tables with periodic wrap.
+ */
+orc_geo *orc_geo_create(const int L[4], const double theta[4])
+{
+   orc_geo *g = (orc_geo *)calloc(1, sizeof(orc_geo));
+   long t, x, y, z, V;
+   if (!g) return NULL;
+   memcpy(g->L, L, sizeof(g->L));
+   memcpy(g->theta, theta, sizeof(g->theta));
+   V = (long)L[0] * L[1] * L[2] * L[3];
+   g->V = V;
+   g->plus = (long *)malloc(sizeof(long) * 4 * V);
+   g->minus = (long *)malloc(sizeof(long) * 4 * V);
+   if (!g->plus || !g->minus) { orc_geo_destroy(g); return NULL; }
+#define LEX(t_, x_, y_, z_) ((t_) + L[0] * ((x_) + L[1] * ((y_) + (long)L[2] * (z_))))
+   for (z = 0; z < L[3]; z++)
+      for (y = 0; y < L[2]; y++)
+         for (x = 0; x < L[1]; x++)
+            for (t = 0; t < L[0]; t++) {
+               long l = LEX(t, x, y, z);
+               g->plus[4*l + 0] = LEX((t + 1) % L[0], x, y, z);
+               g->plus[4*l + 1] = LEX(t, (x + 1) % L[1], y, z);
+               g->plus[4*l + 2] = LEX(t, x, (y + 1) % L[2], z);
+               g->plus[4*l + 3] = LEX(t, x, y, (z + 1) % L[3]);
+               g->minus[4*l + 0] = LEX((t - 1 + L[0]) % L[0], x, y, z);
+               g->minus[4*l + 1] = LEX(t, (x - 1 + L[1]) % L[1], y, z);
+               g->minus[4*l + 2] = LEX(t, x, (y - 1 + L[2]) % L[2], z);
+               g->minus[4*l + 3] = LEX(t, x, y, (z - 1 + L[3]) % L[3]);
+            }
+#undef LEX
+   return g;
+}
+
+void orc_geo_destroy(orc_geo *g)
+{
+   if (!g) return;
+   free(g->plus);
+   free(g->minus);
+   free(g);
+}
+
+long orc_geo_volume(const orc_geo *g) { return g->V; }
+
+/* --------------------------------------------------- spin projection tables
+ * The chiral basis of lib/src/qcd_gamma.c:46-88.  For (1 + s*gamma_mu):
+ *   h0 = psi0 + s*A[mu]*psi[P[mu]],  h1 = psi1 + s*B[mu]*psi[Q[mu]]
+ *   spin2 = s*C[mu]*h[R[mu]],        spin3 = s*D[mu]*h[W[mu]]
+ * (qcd_wilson.h:17-387).  Factors are powers of i.
+ */
+static const int PA[4] = {2, 3, 2, 3}, PB[4] = {2, 3, 0, 1};
+static const int PC[4] = {2, 1, 0, 1}, PD[4] = {2, 1, 2, 3};
+static const int PP[4] = {2, 3, 3, 2}, PQ[4] = {3, 2, 2, 3};
+static const int PR[4] = {0, 1, 1, 0}, PW[4] = {1, 0, 0, 1};
+
+static inline cplx ipow(cplx a, int p)
+{
+   cplx r;
+   switch (p & 3) {
+   case 0: r = a; break;
+   case 1: r.re = -a.im; r.im = a.re; break;
+   case 2: r.re = -a.re; r.im = -a.im; break;
+   default: r.re = a.im; r.im = -a.re; break;
+   }
+   return r;
+}
+
+/* half spinor h[2][3] = upper spins of (1 + s gamma_mu) psi */
+static void project_half(cplx h[2][3], const cplx psi[4][3], int mu, int s)
+{
+   int c, sp = (s > 0) ? 0 : 2;
+   for (c = 0; c < 3; c++) {
+      h[0][c] = cadd(psi[0][c], ipow(psi[PP[mu]][c], PA[mu] + sp));
+      h[1][c] = cadd(psi[1][c], ipow(psi[PQ[mu]][c], PB[mu] + sp));
+   }
+}
+
+/* full spinor from a half spinor of (1 + s gamma_mu) (qcd_wilson.h:539-564) */
+static void reconstruct(cplx f[4][3], const cplx h[2][3], int mu, int s)
+{
+   int c, sp = (s > 0) ? 0 : 2;
+   for (c = 0; c < 3; c++) {
+      f[0][c] = h[0][c];
+      f[1][c] = h[1][c];
+      f[2][c] = ipow(h[PR[mu]][c], PC[mu] + sp);
+      f[3][c] = ipow(h[PW[mu]][c], PD[mu] + sp);
+   }
+}
+
+/* x = U y or U^+ y for nspin colour vectors (qcd_wilson.h:419-509) */
+static void link_mul(cplx x[][3], const cplx U[3][3], const cplx y[][3], int nspin, int dag)
+{
+   int s, r, k;
+   for (s = 0; s < nspin; s++)
+      for (r = 0; r < 3; r++) {
+         cplx acc = {0.0, 0.0};
+         for (k = 0; k < 3; k++) {
+            cplx uu = dag ? conjc(U[k][r]) : U[r][k];
+            acc = cadd(acc, cmul(uu, y[s][k]));
+         }
+         x[s][r] = acc;
+      }
+}
+
+/* hopping phases e^{+-i theta_mu / L_mu} (lib/src/qcd_wilson.c:172-186) */
+static void phases(const orc_geo *g, cplx pp[4], cplx mp[4], int periodic[4])
+{
+   int b;
+   for (b = 0; b < 4; b++) {
+      if (g->theta[b] != 0.0) {
+         double a = g->theta[b] / (double)g->L[b];
+         pp[b].re = cos(a); pp[b].im = sin(a);
+         mp[b].re = cos(a); mp[b].im = -sin(a);
+         periodic[b] = 0;
+      } else {
+         pp[b].re = mp[b].re = 1.0;
+         pp[b].im = mp[b].im = 0.0;
+         periodic[b] = 1;
+      }
+   }
+}
+
+/* (m+4 + i tm gamma5) psi (qcd_wilson.h:389-417) */
+static void mass_terms(cplx d[4][3], const cplx psi[4][3], double mp4, double tm)
+{
+   int s, c;
+   for (s = 0; s < 4; s++) {
+      double g = (s < 2) ? tm : -tm;
+      for (c = 0; c < 3; c++) {
+         d[s][c].re = mp4 * psi[s][c].re - g * psi[s][c].im;
+         d[s][c].im = mp4 * psi[s][c].im + g * psi[s][c].re;
+      }
+   }
+}
+
+/* ------------------------------------------------- Dslash, half-spinor form
+ * lib/src/qcd_wilson.c:114-336.  For each mu: backward hop
+ * U_mu(x-mu)^+ (1+gamma_mu) psi(x-mu) times e^{-i theta/L}, forward hop
+ * U_mu(x) (1-gamma_mu) psi(x+mu) times e^{+i theta/L}; then
+ * out = diag - 1/2 (sum in the order back0, fwd0, back1, fwd1, ...).
+ */
+void orc_dslash_half(const orc_geo *g, double *out, const double *in, const double *u, double m, double tm)
+{
+   cplx pp[4], mp[4];
+   int periodic[4];
+   long l;
+   const double mp4 = 4.0 + m;
+   phases(g, pp, mp, periodic);
+#pragma omp parallel for schedule(static)
+   for (l = 0; l < g->V; l++) {
+      cplx hop[8][2][3];
+      cplx diag[4][3];
+      const cplx (*psi)[3];
+      const cplx (*U)[3];
+      cplx h[2][3];
+      int mu, n, s, c;
+      for (mu = 0; mu < 4; mu++) {
+         long lm = g->minus[4*l + mu], lp = g->plus[4*l + mu];
+         psi = (const cplx (*)[3])(in + 24 * lm);
+         project_half(h, psi, mu, +1);
+         U = (const cplx (*)[3])(u + 72 * lm + 18 * mu);
+         link_mul(hop[2*mu], U, h, 2, 1);
+         psi = (const cplx (*)[3])(in + 24 * lp);
+         project_half(h, psi, mu, -1);
+         U = (const cplx (*)[3])(u + 72 * l + 18 * mu);
+         link_mul(hop[2*mu + 1], U, h, 2, 0);
+         if (!periodic[mu])
+            for (s = 0; s < 2; s++)
+               for (c = 0; c < 3; c++) {
+                  hop[2*mu][s][c] = cmul(hop[2*mu][s][c], mp[mu]);
+                  hop[2*mu + 1][s][c] = cmul(hop[2*mu + 1][s][c], pp[mu]);
+               }
+      }
+      mass_terms(diag, (const cplx (*)[3])(in + 24 * l), mp4, tm);
+      {
+         cplx full[8][4][3];
+         cplx *o = (cplx *)(out + 24 * l);
+         for (n = 0; n < 8; n++) reconstruct(full[n], hop[n], n >> 1, (n & 1) ? -1 : 1);
+         for (s = 0; s < 4; s++)
+            for (c = 0; c < 3; c++) {
+               double re = 0.0, im = 0.0;
+               for (n = 0; n < 8; n++) { re += full[n][s][c].re; im += full[n][s][c].im; }
+               o[3*s + c].re = diag[s][c].re - 0.5 * re;
+               o[3*s + c].im = diag[s][c].im - 0.5 * im;
+            }
+      }
+   }
+}
+
+/* ------------------------------------------- Dslash, full-spinor matrix form
+ * lib/src/qcd_wilson.c:570-792: (1 +- gamma_mu) applied as a 4x4 matrix on the
+ * whole spinor, the link on all four spins.  The matrices are written out from
+ * the gamma table of lib/src/qcd_gamma.c:46-74 (gamma_0..gamma_3).
+ */
+static const double GAMMA_RE[4][4][4] = {
+   {{0,0,-1,0},{0,0,0,-1},{-1,0,0,0},{0,-1,0,0}},
+   {{0,0,0,0},{0,0,0,0},{0,0,0,0},{0,0,0,0}},
+   {{0,0,0,-1},{0,0,1,0},{0,1,0,0},{-1,0,0,0}},
+   {{0,0,0,0},{0,0,0,0},{0,0,0,0},{0,0,0,0}}};
+static const double GAMMA_IM[4][4][4] = {
+   {{0,0,0,0},{0,0,0,0},{0,0,0,0},{0,0,0,0}},
+   {{0,0,0,-1},{0,0,-1,0},{0,1,0,0},{1,0,0,0}},
+   {{0,0,0,0},{0,0,0,0},{0,0,0,0},{0,0,0,0}},
+   {{0,0,-1,0},{0,0,0,1},{1,0,0,0},{0,-1,0,0}}};
+
+static void one_pm_gamma(cplx r[4][3], const cplx v[4][3], int mu, int s)
+{
+   int a, b, c;
+   for (a = 0; a < 4; a++)
+      for (c = 0; c < 3; c++) {
+         cplx acc = v[a][c];
+         for (b = 0; b < 4; b++) {
+            cplx gm = {s * GAMMA_RE[mu][a][b], s * GAMMA_IM[mu][a][b]};
+            acc = cadd(acc, cmul(gm, v[b][c]));
+         }
+         r[a][c] = acc;
+      }
+}
+
+void orc_dslash_full(const orc_geo *g, double *out, const double *in, const double *u, double m, double tm)
+{
+   cplx pp[4], mp[4];
+   int periodic[4];
+   long l;
+   const double mp4 = 4.0 + m;
+   phases(g, pp, mp, periodic);
+#pragma omp parallel for schedule(static)
+   for (l = 0; l < g->V; l++) {
+      cplx hop[8][4][3], proj[4][3], diag[4][3];
+      int mu, n, s, c;
+      for (mu = 0; mu < 4; mu++) {
+         long lm = g->minus[4*l + mu], lp = g->plus[4*l + mu];
+         one_pm_gamma(proj, (const cplx (*)[3])(in + 24 * lm), mu, +1);
+         link_mul(hop[2*mu], (const cplx (*)[3])(u + 72 * lm + 18 * mu), proj, 4, 1);
+         one_pm_gamma(proj, (const cplx (*)[3])(in + 24 * lp), mu, -1);
+         link_mul(hop[2*mu + 1], (const cplx (*)[3])(u + 72 * l + 18 * mu), proj, 4, 0);
+         if (!periodic[mu])
+            for (s = 0; s < 4; s++)
+               for (c = 0; c < 3; c++) {
+                  hop[2*mu][s][c] = cmul(hop[2*mu][s][c], mp[mu]);
+                  hop[2*mu + 1][s][c] = cmul(hop[2*mu + 1][s][c], pp[mu]);
+               }
+      }
+      mass_terms(diag, (const cplx (*)[3])(in + 24 * l), mp4, tm);
+      {
+         cplx *o = (cplx *)(out + 24 * l);
+         for (s = 0; s < 4; s++)
+            for (c = 0; c < 3; c++) {
+               double re = 0.0, im = 0.0;
+               for (n = 0; n < 8; n++) { re += hop[n][s][c].re; im += hop[n][s][c].im; }
+               o[3*s + c].re = diag[s][c].re - 0.5 * re;
+               o[3*s + c].im = diag[s][c].im - 0.5 * im;
+            }
+      }
+   }
+}
+
+/* Q = i gamma5 D: spins 0,1 times i, spins 2,3 times -i (qcd_wilson.c:795-814) */
+void orc_qtm(const orc_geo *g, double *out, const double *in, const double *u, double m, double tm)
+{
+   long l;
+   orc_dslash_full(g, out, in, u, m, tm);
+#pragma omp parallel for schedule(static)
+   for (l = 0; l < g->V; l++) {
+      int s, c;
+      cplx *o = (cplx *)(out + 24 * l);
+      for (s = 0; s < 4; s++)
+         for (c = 0; c < 3; c++) o[3*s + c] = ipow(o[3*s + c], s < 2 ? 1 : 3);
+   }
+}
+
+/* ------------------------------------------------------------ Gauss smearing
+ * lib/src/qcd_smearing.c:25-107: on slice t, v2 = sum_{nu=1..3} U_nu(x) v(x+nu)
+ * + U_nu(x-nu)^+ v(x-nu); v2 *= alpha on the slice; v += v2 everywhere (v2 is
+ * zero off the slice); v *= 1/(1+6 alpha) on the slice.
+ */
+static void gauss_hop_site(const orc_geo *g, cplx acc[4][3], const double *v, const double *u, long i)
+{
+   int nu, s, c;
+   for (s = 0; s < 4; s++)
+      for (c = 0; c < 3; c++) acc[s][c].re = acc[s][c].im = 0.0;
+   for (nu = 1; nu < 4; nu++) {
+      cplx up[4][3], ud[4][3];
+      long ip = g->plus[4*i + nu], im = g->minus[4*i + nu];
+      link_mul(up, (const cplx (*)[3])(u + 72 * i + 18 * nu), (const cplx (*)[3])(v + 24 * ip), 4, 0);
+      link_mul(ud, (const cplx (*)[3])(u + 72 * im + 18 * nu), (const cplx (*)[3])(v + 24 * im), 4, 1);
+      for (s = 0; s < 4; s++)
+         for (c = 0; c < 3; c++) {
+            acc[s][c].re += up[s][c].re + ud[s][c].re;
+            acc[s][c].im += up[s][c].im + ud[s][c].im;
+         }
+   }
+}
+
+void orc_gauss3d(const orc_geo *g, double *v, const double *u, double alpha, int t)
+{
+   const double nrm = 1.0 / (1.0 + alpha * 6.0);
+   long V3 = g->V / g->L[0], j;
+   double *v2 = (double *)malloc(sizeof(double) * 24 * V3);
+   if (!v2) return;
+#pragma omp parallel for schedule(static)
+   for (j = 0; j < V3; j++) {
+      cplx acc[4][3];
+      gauss_hop_site(g, acc, v, u, j * g->L[0] + t);
+      memcpy(v2 + 24 * j, acc, sizeof(acc));
+   }
+   for (j = 0; j < V3; j++) {
+      double *vs = v + 24 * (j * g->L[0] + t), *hs = v2 + 24 * j;
+      int k;
+      for (k = 0; k < 24; k++) vs[k] = (vs[k] + hs[k] * alpha) * nrm;
+   }
+   free(v2);
+}
+
+/* lib/src/qcd_smearing.c:116-199: the same on every time-slice at once */
+void orc_gauss3d_all(const orc_geo *g, double *v, const double *u, double alpha)
+{
+   const double nrm = 1.0 / (1.0 + alpha * 6.0);
+   long i;
+   double *v2 = (double *)malloc(sizeof(double) * 24 * g->V);
+   if (!v2) return;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < g->V; i++) {
+      cplx acc[4][3];
+      gauss_hop_site(g, acc, v, u, i);
+      memcpy(v2 + 24 * i, acc, sizeof(acc));
+   }
+   for (i = 0; i < 24 * g->V; i++) v[i] = (v[i] + v2[i] * alpha) * nrm;
+   free(v2);
+}
+
+/* ------------------------------------------------- vector BLAS, qcd_blas.c */
+void orc_scale(double *v, double r, long V) { long i; for (i = 0; i < 24 * V; i++) v[i] *= r; }   /* :301-308 */
+void orc_sub(double *d, const double *a, const double *b, long V) { long i; for (i = 0; i < 24 * V; i++) d[i] = a[i] - b[i]; } /* :345-354 */
+void orc_add(double *s, const double *a, const double *b, long V) { long i; for (i = 0; i < 24 * V; i++) s[i] = a[i] + b[i]; } /* :403-412 */
+
+/* out = a*x + y (:414-432); safe when out aliases y */
+void orc_axpy(double *out, double are, double aim, const double *x, const double *y, long V)
+{
+   long i;
+   for (i = 0; i < 24 * V; i += 2) {
+      double xr = x[i], xi = x[i + 1];
+      out[i] = y[i] + xr * are - xi * aim;
+      out[i + 1] = y[i + 1] + xi * are + xr * aim;
+   }
+}
+
+void orc_mulc(double *v, double are, double aim, long V)   /* :237-245 */
+{
+   long i;
+   for (i = 0; i < 24 * V; i += 2) {
+      double re = v[i] * are - v[i + 1] * aim;
+      v[i + 1] = v[i] * aim + v[i + 1] * are;
+      v[i] = re;
+   }
+}
+
+/* sum conj(a) b, sequential FP64 (:456-475) */
+void orc_dot(double out[2], const double *a, const double *b, long V)
+{
+   long i;
+   double re = 0.0, im = 0.0;
+   for (i = 0; i < 12 * V; i++) {
+      re += a[2*i] * b[2*i] + a[2*i + 1] * b[2*i + 1];
+      im += a[2*i] * b[2*i + 1] - a[2*i + 1] * b[2*i];
+   }
+   out[0] = re;
+   out[1] = im;
+}
+
+/* sqrt(sum x^2), sequential FP64 (:478-491) */
+double orc_norm(const double *v, long V)
+{
+   long i;
+   double s = 0.0;
+   for (i = 0; i < 24 * V; i++) s += v[i] * v[i];
+   return sqrt(s);
+}
+
+/* ------------------------------------------------------------------- solver */
+static void apply_op(const orc_geo *g, int op, double *out, const double *in, const double *u, double kappa, double mu)
+{
+   double m = 1.0 / (2.0 * kappa) - 4.0;
+   if (op == 0) orc_qtm(g, out, in, u, m, mu);
+   else orc_dslash_half(g, out, in, u, m, mu);
+}
+
+/* invert.c:20-113 (algorithm spec gcr.m:3-48) */
+int orc_gcr(const orc_geo *g, double *sol, const double *src, const double *u,
+            double kappa, double mu, int n, int op, int keep_q1)
+{
+   const long V = g->V, N = 24 * V;
+   double **chi = (double **)calloc(n, sizeof(double *));
+   double **res = (double **)calloc(n, sizeof(double *));
+   cplx *a = (cplx *)calloc((size_t)n * n, sizeof(cplx));
+   cplx *c = (cplx *)calloc(n, sizeof(cplx));
+   int i, j, k, l, rc = 0;
+   double d[2];
+   if (!chi || !res || !a || !c) { rc = 1; goto done; }
+   for (i = 0; i < n; i++) {
+      chi[i] = (double *)malloc(sizeof(double) * N);
+      res[i] = (double *)malloc(sizeof(double) * N);
+      if (!chi[i] || !res[i]) { rc = 1; goto done; }
+   }
+#define A(k_, j_) a[(k_) * n + (j_)]
+   memcpy(res[0], src, sizeof(double) * N);
+   apply_op(g, op, chi[0], res[0], u, kappa, mu);
+   A(0, 0).re = 1.0 / orc_norm(chi[0], V);
+   A(0, 0).im = 0.0;
+   orc_scale(chi[0], A(0, 0).re, V);
+   orc_dot(d, chi[0], src, V);
+   c[0].re = d[0]; c[0].im = d[1];
+   memset(sol, 0, sizeof(double) * N);
+   if (!keep_q1) orc_axpy(sol, c[0].re * A(0, 0).re, c[0].im * A(0, 0).re, res[0], sol, V);
+   for (k = 1; k < n; k++) {
+      orc_axpy(res[k], -c[k - 1].re, -c[k - 1].im, chi[k - 1], res[k - 1], V);
+      apply_op(g, op, chi[k], res[k], u, kappa, mu);
+      for (j = 0; j < k; j++) {          /* modified Gram-Schmidt, invert.c:68-73 */
+         orc_dot(d, chi[j], chi[k], V);
+         A(k, j).re = -d[0];
+         A(k, j).im = -d[1];
+         orc_axpy(chi[k], A(k, j).re, A(k, j).im, chi[j], chi[k], V);
+      }
+      A(k, k).re = 1.0 / orc_norm(chi[k], V);
+      A(k, k).im = 0.0;
+      if (isnan(A(k, k).re)) { rc = 1; goto done; }
+      orc_scale(chi[k], A(k, k).re, V);
+      for (j = 0; j < k; j++) { A(k, j).re *= A(k, k).re; A(k, j).im *= A(k, k).re; }
+      for (j = 0; j < k; j++) {           /* invert.c:87-94 */
+         A(k, j).re *= A(j, j).re;
+         A(k, j).im *= A(j, j).re;
+         for (l = j + 1; l < k; l++) A(k, j) = cadd(A(k, j), cmul(A(k, l), A(l, j)));
+      }
+      orc_dot(d, chi[k], src, V);
+      c[k].re = d[0]; c[k].im = d[1];
+      for (j = 0; j <= k; j++) {          /* invert.c:100-103 */
+         cplx f = cmul(c[k], A(k, j));
+         orc_axpy(sol, f.re, f.im, res[j], sol, V);
+      }
+   }
+#undef A
+done:
+   if (chi) for (i = 0; i < n; i++) free(chi[i]);
+   if (res) for (i = 0; i < n; i++) free(res[i]);
+   free(chi); free(res); free(a); free(c);
+   return rc;
+}
+
+/* invert.c:255-294 with maxRes and maxIter as parameters */
+int orc_invert(const orc_geo *g, double *x, const double *b, const double *u,
+               double kappa, double mu, int n, int op, int keep_q1,
+               double max_res, int max_outer, int *outer_out, double *relres_out)
+{
+   const long V = g->V, N = 24 * V;
+   double *src = (double *)malloc(sizeof(double) * N);
+   double *res = (double *)malloc(sizeof(double) * N);
+   double *cor = (double *)malloc(sizeof(double) * N);
+   double normsrc, normres;
+   int iter = 1, rc = 0;
+   if (!src || !res || !cor) { free(src); free(res); free(cor); return 1; }
+   memcpy(src, b, sizeof(double) * N);
+   if (op == 0) {          /* Q x = i gamma5 b is the same system as D x = b */
+      long l;
+      for (l = 0; l < V; l++) {
+         int s, c;
+         cplx *p = (cplx *)(src + 24 * l);
+         for (s = 0; s < 4; s++)
+            for (c = 0; c < 3; c++) p[3*s + c] = ipow(p[3*s + c], s < 2 ? 1 : 3);
+      }
+   }
+   normsrc = orc_norm(src, V);
+   if (orc_gcr(g, x, src, u, kappa, mu, n, op, keep_q1)) rc = 1;
+   apply_op(g, op, res, x, u, kappa, mu);
+   orc_sub(res, src, res, V);
+   normres = orc_norm(res, V) / normsrc;
+   while (!rc && normres > max_res && iter < max_outer) {
+      if (orc_gcr(g, cor, res, u, kappa, mu, n, op, keep_q1)) { rc = 1; break; }
+      orc_add(x, x, cor, V);
+      apply_op(g, op, res, x, u, kappa, mu);
+      orc_sub(res, src, res, V);
+      normres = orc_norm(res, V) / normsrc;
+      iter++;
+   }
+   if (outer_out) *outer_out = iter;
+   if (relres_out) *relres_out = normres;
+   free(src); free(res); free(cor);
+   if (rc) return rc;
+   return normres <= max_res ? 0 : 2;
+}
+
+double orc_b_minus_Dx(const orc_geo *g, const double *x, const double *b, const double *u,
+                      double kappa, double mu)
+{
+   const long V = g->V;
+   double *dx = (double *)malloc(sizeof(double) * 24 * V), r;
+   if (!dx) return -1.0;
+   orc_dslash_half(g, dx, x, u, 1.0 / (2.0 * kappa) - 4.0, mu);
+   orc_sub(dx, dx, b, V);
+   r = orc_norm(dx, V) / orc_norm(b, V);
+   free(dx);
+   return r;
+}
+
+/* ------------------------------------------------------- synthetic inputs
+ * Counter-based generator: draw k of stream s under seed is
+ *   mix(key(seed, s) + k * 0x632BE59BD9B4E019), key = mix(seed) ^ s * 0xD1B54A32D192ED03
+ * with mix the splitmix64 finaliser; uniforms in (0,1) from the top 53 bits;
+ * normals by Box-Muller.  The product library implements the same recipe.
+ */
+static inline uint64_t mix64(uint64_t z)
+{
+   z += 0x9E3779B97F4A7C15ULL;
+   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+   return z ^ (z >> 31);
+}
+
+static inline double uniform01(uint64_t key, uint64_t k)
+{
+   uint64_t r = mix64(key + k * 0x632BE59BD9B4E019ULL);
+   return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+static void normals(double *out, int n, uint64_t seed, uint64_t stream)
+{
+   uint64_t key = mix64(seed) ^ (stream * 0xD1B54A32D192ED03ULL);
+   int i;
+   for (i = 0; i < n; i += 2) {
+      double u1 = uniform01(key, (uint64_t)i), u2 = uniform01(key, (uint64_t)i + 1);
+      double r = sqrt(-2.0 * log(u1));
+      out[i] = r * cos(6.283185307179586 * u2);
+      out[i + 1] = r * sin(6.283185307179586 * u2);
+   }
+}
+
+static void mat_mul3(cplx c[3][3], const cplx a[3][3], const cplx b[3][3])
+{
+   int i, j, k;
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) {
+         cplx s = {0.0, 0.0};
+         for (k = 0; k < 3; k++) s = cadd(s, cmul(a[i][k], b[k][j]));
+         c[i][j] = s;
+      }
+}
+
+/* U = reunitarize(exp(i eps H)), H = (A + A^+)/2 made traceless,
+ * A_ij = N(0,1) + i N(0,1).  exp by Taylor series (degree 16) on X/2^s with
+ * |X/2^s|_inf <= 1/4, then s squarings; reunitarize by Gram-Schmidt of rows
+ * 0,1 and row2 = conj(row0 x row1). */
+static void make_link(cplx U[3][3], const double g[18], double eps)
+{
+   cplx H[3][3], X[3][3], E[3][3], T[3][3];
+   double tr, nrm = 0.0, scale = 1.0;
+   int i, j, k, s = 0;
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) {
+         H[i][j].re = 0.5 * (g[2*(3*i + j)] + g[2*(3*j + i)]);
+         H[i][j].im = 0.5 * (g[2*(3*i + j) + 1] - g[2*(3*j + i) + 1]);
+      }
+   tr = (H[0][0].re + H[1][1].re + H[2][2].re) / 3.0;
+   for (i = 0; i < 3; i++) { H[i][i].re -= tr; H[i][i].im = 0.0; }
+   for (i = 0; i < 3; i++) {
+      double row = 0.0;
+      for (j = 0; j < 3; j++) {
+         X[i][j].re = -eps * H[i][j].im;   /* X = i eps H */
+         X[i][j].im = eps * H[i][j].re;
+         row += fabs(X[i][j].re) + fabs(X[i][j].im);
+      }
+      if (row > nrm) nrm = row;
+   }
+   while (nrm * scale > 0.25) { scale *= 0.5; s++; }
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) { X[i][j].re *= scale; X[i][j].im *= scale; }
+   /* Horner: E = I + X/1 (I + X/2 (I + ... (I + X/16))) */
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) { E[i][j].re = (i == j) ? 1.0 : 0.0; E[i][j].im = 0.0; }
+   for (k = 16; k >= 1; k--) {
+      mat_mul3(T, X, E);
+      for (i = 0; i < 3; i++)
+         for (j = 0; j < 3; j++) {
+            E[i][j].re = T[i][j].re / (double)k + ((i == j) ? 1.0 : 0.0);
+            E[i][j].im = T[i][j].im / (double)k;
+         }
+   }
+   for (k = 0; k < s; k++) {
+      mat_mul3(T, E, E);
+      memcpy(E, T, sizeof(E));
+   }
+   {
+      double n0 = 0.0, n1 = 0.0;
+      cplx p = {0.0, 0.0};
+      for (j = 0; j < 3; j++) n0 += E[0][j].re * E[0][j].re + E[0][j].im * E[0][j].im;
+      n0 = sqrt(n0);
+      for (j = 0; j < 3; j++) { U[0][j].re = E[0][j].re / n0; U[0][j].im = E[0][j].im / n0; }
+      for (j = 0; j < 3; j++) p = cadd(p, cmul(conjc(U[0][j]), E[1][j]));
+      for (j = 0; j < 3; j++) {
+         cplx q = cmul(p, U[0][j]);
+         U[1][j].re = E[1][j].re - q.re;
+         U[1][j].im = E[1][j].im - q.im;
+      }
+      for (j = 0; j < 3; j++) n1 += U[1][j].re * U[1][j].re + U[1][j].im * U[1][j].im;
+      n1 = sqrt(n1);
+      for (j = 0; j < 3; j++) { U[1][j].re /= n1; U[1][j].im /= n1; }
+      for (j = 0; j < 3; j++) {
+         int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+         cplx x = cmul(U[0][j1], U[1][j2]), y = cmul(U[0][j2], U[1][j1]);
+         U[2][j].re = x.re - y.re;
+         U[2][j].im = -(x.im - y.im);
+      }
+   }
+}
+
+void orc_gauge_generate(double *u, long V, double eps, uint64_t seed)
+{
+   long i;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 4 * V; i++) {
+      double g[18];
+      normals(g, 18, seed, (uint64_t)i);
+      make_link((cplx (*)[3])(u + 18 * i), g, eps);
+   }
+}
+
+void orc_spinor_gaussian(double *v, long V, uint64_t seed)
+{
+   long i;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < V; i++) normals(v + 24 * i, 24, seed ^ 0x5350494E4F52ULL, (uint64_t)i);
+}
+
+void orc_point_source(double *v, const int L[4], int t, int x, int y, int z, int spin, int col)
+{
+   long V = (long)L[0] * L[1] * L[2] * L[3];
+   long l = t + (long)L[0] * (x + (long)L[1] * (y + (long)L[2] * z));
+   memset(v, 0, sizeof(double) * 24 * V);
+   v[24 * l + 6 * spin + 2 * col] = 1.0;
+}
+
+/* lib/src/qcd_observables.c:15-36 */
+double orc_plaquette(const orc_geo *g, const double *u)
+{
+   double sum = 0.0;
+   long l;
+   for (l = 0; l < g->V; l++) {
+      int mu, nu, i;
+      for (mu = 0; mu < 3; mu++)
+         for (nu = mu + 1; nu < 4; nu++) {
+            const cplx (*U1)[3] = (const cplx (*)[3])(u + 72 * l + 18 * mu);
+            const cplx (*U2)[3] = (const cplx (*)[3])(u + 72 * g->plus[4*l + mu] + 18 * nu);
+            const cplx (*U3)[3] = (const cplx (*)[3])(u + 72 * g->plus[4*l + nu] + 18 * mu);
+            const cplx (*U4)[3] = (const cplx (*)[3])(u + 72 * l + 18 * nu);
+            cplx P[3][3], T[3][3], A3[3][3], A4[3][3];
+            int r, c;
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) { A3[r][c] = conjc(U3[c][r]); A4[r][c] = conjc(U4[c][r]); }
+            mat_mul3(P, U1, U2);
+            mat_mul3(T, P, A3);
+            mat_mul3(P, T, A4);
+            for (i = 0; i < 3; i++) sum += P[i][i].re;
+         }
+   }
+   return sum / ((double)g->V * 3.0 * 6.0);
+}

@@ -1,0 +1,240 @@
+// blas.hip -- vector BLAS and deterministic FP64 reductions on spinor fields.
+//
+// Reductions: a fixed grid (depending only on the lattice volume) writes one
+// partial per workgroup; a single workgroup then sums the partials in a fixed
+// order.  Results are therefore bitwise reproducible run to run and do not
+// depend on what else shares the GPU (lib/src/qcd_blas.c:456-491 sums
+// sequentially; the difference is rounding of order 1e-16 * sqrt(V)).
+#include "blas_common.h"
+
+namespace qcdhip {
+
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_norm2(SpinorIn<P> v, unsigned V, double *partials) {
+  using R = typename Prec<P>::R;
+  double acc = 0.0;
+  for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < V; s += gridDim.x * kBlock) {
+    Cx<R> x[12];
+    v.load(s, x);
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc += (double)x[k].re * (double)x[k].re + (double)x[k].im * (double)x[k].im;
+  }
+  block_reduce_store(acc, partials, blockIdx.x);
+}
+
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_dot(SpinorIn<P> a, SpinorIn<P> b, unsigned V, double *partials) {
+  using R = typename Prec<P>::R;
+  double re = 0.0, im = 0.0;
+  for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < V; s += gridDim.x * kBlock) {
+    Cx<R> x[12], y[12];
+    a.load(s, x);
+    b.load(s, y);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      re += (double)x[k].re * (double)y[k].re + (double)x[k].im * (double)y[k].im;
+      im += (double)x[k].re * (double)y[k].im - (double)x[k].im * (double)y[k].re;
+    }
+  }
+  block_reduce_store(re, partials, blockIdx.x);
+  __syncthreads();
+  block_reduce_store(im, partials + gridDim.x, blockIdx.x);
+}
+
+// y = a x + y
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_axpy(SpinorIn<P> x, SpinorIn<P> yin, SpinorOut<P> y, double are,
+                                                 double aim, unsigned V) {
+  using R = typename Prec<P>::R;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= V) return;
+  Cx<R> xv[12], yv[12];
+  x.load(s, xv);
+  yin.load(s, yv);
+  const R ar = (R)are, ai = (R)aim;
+#pragma unroll
+  for (int k = 0; k < 12; k++)
+    yv[k] = Cx<R>{yv[k].re + ar * xv[k].re - ai * xv[k].im, yv[k].im + ar * xv[k].im + ai * xv[k].re};
+  y.store(s, yv);
+}
+
+// out = a x + b y on every site, or only on time-slice t0 when t0 >= 0
+// (other sites of out are left untouched); y may be absent (b = 0).
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_lincomb(SpinorIn<P> x, SpinorIn<P> y, bool has_y, SpinorOut<P> out,
+                                                    double are, double aim, double bre, double bim, unsigned V,
+                                                    int t0, unsigned L0) {
+  using R = typename Prec<P>::R;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= V) return;
+  if (t0 >= 0 && (int)(s % L0) != t0) return;
+  Cx<R> xv[12], yv[12];
+  x.load(s, xv);
+  if (has_y) y.load(s, yv);
+  const R ar = (R)are, ai = (R)aim, br = (R)bre, bi = (R)bim;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    Cx<R> o{ar * xv[k].re - ai * xv[k].im, ar * xv[k].im + ai * xv[k].re};
+    if (has_y) {
+      o.re += br * yv[k].re - bi * yv[k].im;
+      o.im += br * yv[k].im + bi * yv[k].re;
+    }
+    xv[k] = o;
+  }
+  out.store(s, xv);
+}
+
+// Sum n_sums interleaved partial arrays (each nparts long) into out[0..n_sums).
+__global__ void __launch_bounds__(kBlock) k_finalize(const double *partials, int nparts, int n_sums, double *out) {
+  __shared__ double sh[kBlock];
+  for (int j = 0; j < n_sums; j++) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) v += partials[(size_t)j * nparts + i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = sh[0];
+    __syncthreads();
+  }
+}
+
+int reduce_grid(long V) {
+  int g = grid_for(V);
+  return g < kMaxReduceBlocks ? g : kMaxReduceBlocks;
+}
+
+int finalize(const double *partials, int nparts, int n_sums, double *out, hipStream_t st) {
+  k_finalize<<<1, kBlock, 0, st>>>(partials, nparts, n_sums, out);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int read_scalars(int n, double *host) {
+  Context *c = ctx();
+  QH_CHECK(hipMemcpyAsync(c->host_scalars, c->scalars, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  QH_CHECK(hipStreamSynchronize(c->stream));
+  std::memcpy(host, c->host_scalars, sizeof(double) * n);
+  return 0;
+}
+
+static int check_rhs(qcd_hip_field f, int r, const char *who) {
+  if (!f || f->kind != QCD_HIP_SPINOR || r < 0 || r >= f->nrhs) {
+    set_error("%s: bad spinor field or rhs index", who);
+    return 1;
+  }
+  return 0;
+}
+
+int lincomb_launch(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx, double bre, double bim,
+                   qcd_hip_field y, int ry, int t0, hipStream_t st) {
+  QH_TRY(check_rhs(out, ro, "lincomb"));
+  QH_TRY(check_rhs(x, rx, "lincomb"));
+  if (y) QH_TRY(check_rhs(y, ry, "lincomb"));
+  if (x->prec != out->prec || x->lat->V != out->lat->V || (y && (y->prec != out->prec || y->lat->V != out->lat->V))) {
+    set_error("lincomb: fields differ in precision or lattice");
+    return 1;
+  }
+  if (t0 >= out->lat->L[0]) {
+    set_error("lincomb: time-slice %d outside 0..%d", t0, out->lat->L[0] - 1);
+    return 1;
+  }
+  const unsigned V = (unsigned)out->lat->V, L0 = (unsigned)out->lat->L[0];
+  const bool hy = y != nullptr;
+  qcd_hip_field yy = hy ? y : x;
+  const int ryy = hy ? ry : rx;
+  switch (out->prec) {
+    case QCD_HIP_F64:
+      k_lincomb<QCD_HIP_F64><<<grid_for(V), kBlock, 0, st>>>(SpinorIn<QCD_HIP_F64>::of(x, rx), SpinorIn<QCD_HIP_F64>::of(yy, ryy), hy, SpinorOut<QCD_HIP_F64>::of(out, ro), are, aim, bre, bim, V, t0, L0);
+      break;
+    case QCD_HIP_F32:
+      k_lincomb<QCD_HIP_F32><<<grid_for(V), kBlock, 0, st>>>(SpinorIn<QCD_HIP_F32>::of(x, rx), SpinorIn<QCD_HIP_F32>::of(yy, ryy), hy, SpinorOut<QCD_HIP_F32>::of(out, ro), are, aim, bre, bim, V, t0, L0);
+      break;
+    default:
+      k_lincomb<QCD_HIP_F16><<<grid_for(V), kBlock, 0, st>>>(SpinorIn<QCD_HIP_F16>::of(x, rx), SpinorIn<QCD_HIP_F16>::of(yy, ryy), hy, SpinorOut<QCD_HIP_F16>::of(out, ro), are, aim, bre, bim, V, t0, L0);
+      break;
+  }
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcdhip
+
+using namespace qcdhip;
+
+extern "C" {
+
+int qcd_hip_norm2(qcd_hip_field v, int rhs, double *out) {
+  QH_TRY(ensure_init());
+  QH_TRY(check_rhs(v, rhs, "norm2"));
+  Context *c = ctx();
+  const unsigned V = (unsigned)v->lat->V;
+  const int g = reduce_grid(V);
+  switch (v->prec) {
+    case QCD_HIP_F64: k_norm2<QCD_HIP_F64><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F64>::of(v, rhs), V, c->partials); break;
+    case QCD_HIP_F32: k_norm2<QCD_HIP_F32><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F32>::of(v, rhs), V, c->partials); break;
+    default: k_norm2<QCD_HIP_F16><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F16>::of(v, rhs), V, c->partials); break;
+  }
+  QH_CHECK(hipGetLastError());
+  QH_TRY(finalize(c->partials, g, 1, c->scalars, c->stream));
+  return read_scalars(1, out);
+}
+
+int qcd_hip_dot(qcd_hip_field a, int ra, qcd_hip_field b, int rb, double out[2]) {
+  QH_TRY(ensure_init());
+  QH_TRY(check_rhs(a, ra, "dot"));
+  QH_TRY(check_rhs(b, rb, "dot"));
+  if (a->prec != b->prec || a->lat->V != b->lat->V) {
+    set_error("dot: fields differ in precision or lattice");
+    return 1;
+  }
+  Context *c = ctx();
+  const unsigned V = (unsigned)a->lat->V;
+  const int g = reduce_grid(V);
+  switch (a->prec) {
+    case QCD_HIP_F64: k_dot<QCD_HIP_F64><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F64>::of(a, ra), SpinorIn<QCD_HIP_F64>::of(b, rb), V, c->partials); break;
+    case QCD_HIP_F32: k_dot<QCD_HIP_F32><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F32>::of(a, ra), SpinorIn<QCD_HIP_F32>::of(b, rb), V, c->partials); break;
+    default: k_dot<QCD_HIP_F16><<<g, kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F16>::of(a, ra), SpinorIn<QCD_HIP_F16>::of(b, rb), V, c->partials); break;
+  }
+  QH_CHECK(hipGetLastError());
+  QH_TRY(finalize(c->partials, g, 2, c->scalars, c->stream));
+  return read_scalars(2, out);
+}
+
+int qcd_hip_lincomb(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx, double bre,
+                    double bim, qcd_hip_field y, int ry, int t0) {
+  QH_TRY(ensure_init());
+  QH_TRY(lincomb_launch(out, ro, are, aim, x, rx, bre, bim, y, ry, t0, ctx()->stream));
+  QH_CHECK(hipStreamSynchronize(ctx()->stream));
+  return 0;
+}
+
+int qcd_hip_axpy(qcd_hip_field y, int ry, double are, double aim, qcd_hip_field x, int rx) {
+  QH_TRY(ensure_init());
+  QH_TRY(check_rhs(y, ry, "axpy"));
+  QH_TRY(check_rhs(x, rx, "axpy"));
+  if (x->prec != y->prec || x->lat->V != y->lat->V) {
+    set_error("axpy: fields differ in precision or lattice");
+    return 1;
+  }
+  Context *c = ctx();
+  const unsigned V = (unsigned)y->lat->V;
+  switch (y->prec) {
+    case QCD_HIP_F64:
+      k_axpy<QCD_HIP_F64><<<grid_for(V), kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F64>::of(x, rx), SpinorIn<QCD_HIP_F64>::of(y, ry), SpinorOut<QCD_HIP_F64>::of(y, ry), are, aim, V);
+      break;
+    case QCD_HIP_F32:
+      k_axpy<QCD_HIP_F32><<<grid_for(V), kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F32>::of(x, rx), SpinorIn<QCD_HIP_F32>::of(y, ry), SpinorOut<QCD_HIP_F32>::of(y, ry), are, aim, V);
+      break;
+    default:
+      k_axpy<QCD_HIP_F16><<<grid_for(V), kBlock, 0, c->stream>>>(SpinorIn<QCD_HIP_F16>::of(x, rx), SpinorIn<QCD_HIP_F16>::of(y, ry), SpinorOut<QCD_HIP_F16>::of(y, ry), are, aim, V);
+      break;
+  }
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+// dslash_common.h -- device building blocks shared by the Dslash kernels:
+// storage accessors per precision, the spin-projection table, one hopping
+// direction, epilogues and the block reduction.
+#pragma once
+
+#include "internal.h"
+
+namespace qcdhip {
+
+template <typename R>
+struct Cx {
+  R re, im;
+};
+
+template <int P>
+struct Prec;
+template <>
+struct Prec<QCD_HIP_F64> {
+  using T = double2;
+  using R = double;
+};
+template <>
+struct Prec<QCD_HIP_F32> {
+  using T = float2;
+  using R = float;
+};
+template <>
+struct Prec<QCD_HIP_F16> {
+  using T = __half2;
+  using R = float;
+};
+
+template <typename R, typename T>
+__device__ __forceinline__ Cx<R> to_cx(T v) {
+  if constexpr (sizeof(T) == 4) {
+    float2 f = __half22float2(v);
+    return Cx<R>{(R)f.x, (R)f.y};
+  } else {
+    return Cx<R>{(R)v.x, (R)v.y};
+  }
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ T from_cx(Cx<R> v) {
+  if constexpr (sizeof(T) == 16) {
+    return make_double2((double)v.re, (double)v.im);
+  } else if constexpr (sizeof(T) == 8) {
+    return make_float2((float)v.re, (float)v.im);
+  } else {
+    return __floats2half2_rn((float)v.re, (float)v.im);
+  }
+}
+
+// ------------------------------------------------------------ accessors
+template <int P>
+struct SpinorIn {
+  using T = typename Prec<P>::T;
+  using R = typename Prec<P>::R;
+  const T *__restrict__ p;
+  const float *__restrict__ norm;
+  size_t stride;
+  static SpinorIn of(qcd_hip_field f, int rhs) {
+    SpinorIn s;
+    s.stride = (size_t)f->lat->stride;
+    s.p = (const T *)f->data + (size_t)rhs * 12 * s.stride;
+    s.norm = f->norm ? f->norm + (size_t)rhs * s.stride : nullptr;
+    return s;
+  }
+  __device__ __forceinline__ void load(unsigned s, Cx<R> v[12]) const {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(p[k * stride + s]);
+    if constexpr (P == QCD_HIP_F16) {
+      const R n = norm[s];
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        v[k].re *= n;
+        v[k].im *= n;
+      }
+    }
+  }
+};
+
+template <int P>
+struct SpinorOut {
+  using T = typename Prec<P>::T;
+  using R = typename Prec<P>::R;
+  T *__restrict__ p;
+  float *__restrict__ norm;
+  size_t stride;
+  static SpinorOut of(qcd_hip_field f, int rhs) {
+    SpinorOut s;
+    s.stride = (size_t)f->lat->stride;
+    s.p = (T *)f->data + (size_t)rhs * 12 * s.stride;
+    s.norm = f->norm ? f->norm + (size_t)rhs * s.stride : nullptr;
+    return s;
+  }
+  __device__ __forceinline__ void store(unsigned s, const Cx<R> v[12]) const {
+    if constexpr (P == QCD_HIP_F16) {
+      R mx = R(0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+      const R inv = mx > R(0) ? R(1) / mx : R(0);
+      norm[s] = mx;
+#pragma unroll
+      for (int k = 0; k < 12; k++) p[k * stride + s] = from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv});
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) p[k * stride + s] = from_cx<T>(v[k]);
+    }
+  }
+};
+
+template <int P>
+struct GaugeIn {
+  using T = typename Prec<P>::T;
+  const T *__restrict__ p;
+  size_t stride;
+  static GaugeIn of(qcd_hip_field f) {
+    GaugeIn g;
+    g.stride = (size_t)f->lat->stride;
+    g.p = (const T *)f->data;
+    return g;
+  }
+  template <typename R>
+  __device__ __forceinline__ void load(int mu, unsigned s, Cx<R> u[9]) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = to_cx<R>(p[(size_t)(9 * mu + k) * stride + s]);
+  }
+};
+
+// ------------------------------------------------------------ geometry
+struct DslashArgs {
+  unsigned V;
+  unsigned L[4];
+  size_t stride;
+  double ph_re[4], ph_im[4];
+  int twisted[4];
+  double mp4, tm;
+};
+
+struct Site {
+  unsigned s, t, x, y, z;
+};
+
+__device__ __forceinline__ Site site_coords(unsigned s, const DslashArgs &a) {
+  Site st;
+  st.s = s;
+  unsigned r = s / a.L[0];
+  st.t = s - r * a.L[0];
+  unsigned r2 = r / a.L[1];
+  st.x = r - r2 * a.L[1];
+  st.z = r2 / a.L[2];
+  st.y = r2 - st.z * a.L[2];
+  return st;
+}
+
+// neighbour of st in direction MU: forward (+1) or backward (-1)
+template <int MU, int DIR>
+__device__ __forceinline__ unsigned neighbour(const Site &st, const DslashArgs &a) {
+  unsigned c, len, step;
+  if constexpr (MU == 0) {
+    c = st.t; len = a.L[0]; step = 1;
+  } else if constexpr (MU == 1) {
+    c = st.x; len = a.L[1]; step = a.L[0];
+  } else if constexpr (MU == 2) {
+    c = st.y; len = a.L[2]; step = a.L[0] * a.L[1];
+  } else {
+    c = st.z; len = a.L[3]; step = a.L[0] * a.L[1] * a.L[2];
+  }
+  if constexpr (DIR > 0) return (c == len - 1) ? st.s - (len - 1) * step : st.s + step;
+  else return (c == 0) ? st.s + (len - 1) * step : st.s - step;
+}
+
+// ------------------------------------------------------- projector table
+// Chiral basis of lib/src/qcd_gamma.c:46-88.  (1 + S g_mu) psi has upper spins
+//   h0 = psi_0 + S i^A psi_P,  h1 = psi_1 + S i^B psi_Q
+// and lower spins  psi_2 = S i^C h_R,  psi_3 = S i^D h_W  (qcd_wilson.h:17-387).
+template <int MU>
+struct Proj;
+template <>
+struct Proj<0> { static constexpr int A = 2, B = 2, C = 2, D = 2, P = 2, Q = 3, R = 0, W = 1; };
+template <>
+struct Proj<1> { static constexpr int A = 3, B = 3, C = 1, D = 1, P = 3, Q = 2, R = 1, W = 0; };
+template <>
+struct Proj<2> { static constexpr int A = 2, B = 0, C = 0, D = 2, P = 3, Q = 2, R = 1, W = 0; };
+template <>
+struct Proj<3> { static constexpr int A = 3, B = 1, C = 1, D = 3, P = 2, Q = 3, R = 0, W = 1; };
+
+template <int POW, typename R>
+__device__ __forceinline__ Cx<R> ipow(Cx<R> v) {
+  constexpr int p = POW & 3;
+  if constexpr (p == 0) return v;
+  else if constexpr (p == 1) return Cx<R>{-v.im, v.re};
+  else if constexpr (p == 2) return Cx<R>{-v.re, -v.im};
+  else return Cx<R>{v.im, -v.re};
+}
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return Cx<R>{a.re + b.re, a.im + b.im}; }
+
+// y = U x (DAG = false) or U^+ x (DAG = true) for one colour vector
+template <bool DAG, typename R>
+__device__ __forceinline__ void su3_mul(Cx<R> y[3], const Cx<R> u[9], const Cx<R> x[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    R re = R(0), im = R(0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const Cx<R> m = DAG ? Cx<R>{u[3 * k + r].re, -u[3 * k + r].im} : u[3 * r + k];
+      re += m.re * x[k].re - m.im * x[k].im;
+      im += m.re * x[k].im + m.im * x[k].re;
+    }
+    y[r] = Cx<R>{re, im};
+  }
+}
+
+// accumulate one hop: phase * link * (1 + S g_MU) psi, rebuilt to 4 spins
+template <int MU, int S, bool DAG, typename R>
+__device__ __forceinline__ void hop_accumulate(Cx<R> acc[4][3], const Cx<R> psi[12], const Cx<R> u[9], bool twisted,
+                                               Cx<R> ph) {
+  using PJ = Proj<MU>;
+  constexpr int sp = S > 0 ? 0 : 2;
+  Cx<R> h0[3], h1[3], g0[3], g1[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h0[c] = cadd(psi[c], ipow<PJ::A + sp>(psi[3 * PJ::P + c]));
+    h1[c] = cadd(psi[3 + c], ipow<PJ::B + sp>(psi[3 * PJ::Q + c]));
+  }
+  su3_mul<DAG>(g0, u, h0);
+  su3_mul<DAG>(g1, u, h1);
+  if (twisted) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
+      g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    acc[0][c] = cadd(acc[0][c], g0[c]);
+    acc[1][c] = cadd(acc[1][c], g1[c]);
+    acc[2][c] = cadd(acc[2][c], ipow<PJ::C + sp>(PJ::R == 0 ? g0[c] : g1[c]));
+    acc[3][c] = cadd(acc[3][c], ipow<PJ::D + sp>(PJ::W == 0 ? g0[c] : g1[c]));
+  }
+}
+
+// forward hop U_mu(x)(1-g_mu)psi(x+mu) e^{+i th/L} and backward hop
+// U_mu(x-mu)^+ (1+g_mu) psi(x-mu) e^{-i th/L}
+template <int MU, int PS, int PG, typename R>
+__device__ __forceinline__ void hop_dir(Cx<R> acc[4][3], const DslashArgs &a, const Site &st, const SpinorIn<PS> &in,
+                                        const GaugeIn<PG> &U) {
+  const bool tw = a.twisted[MU] != 0;
+  const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
+  const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
+  Cx<R> psi[12], u[9];
+  {
+    const unsigned sf = neighbour<MU, +1>(st, a);
+    in.load(sf, psi);
+    U.load(MU, st.s, u);
+    hop_accumulate<MU, -1, false>(acc, psi, u, tw, php);
+  }
+  {
+    const unsigned sb = neighbour<MU, -1>(st, a);
+    in.load(sb, psi);
+    U.load(MU, sb, u);
+    hop_accumulate<MU, +1, true>(acc, psi, u, tw, phm);
+  }
+}
+
+// ------------------------------------------------------------ epilogues
+enum { EPI_D = 0, EPI_Q = 1, EPI_RES_D = 2, EPI_RES_Q = 3 };
+
+template <int EPI, int PS, typename R>
+__device__ __forceinline__ void apply_epilogue(Cx<R> o[12], const SpinorIn<PS> &b, unsigned s) {
+  if constexpr (EPI == EPI_Q || EPI == EPI_RES_Q) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = ipow<1>(o[k]);
+#pragma unroll
+    for (int k = 6; k < 12; k++) o[k] = ipow<3>(o[k]);
+  }
+  if constexpr (EPI == EPI_RES_D || EPI == EPI_RES_Q) {
+    Cx<R> bv[12];
+    b.load(s, bv);
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[k] = Cx<R>{bv[k].re - o[k].re, bv[k].im - o[k].im};
+  }
+}
+
+// ------------------------------------------------------------ reduction
+// Sum v over the workgroup (64-wide wavefront shuffles, then LDS) and store
+// it at partials[slot].  Every lane of the workgroup must call this.
+__device__ __forceinline__ void block_reduce_store(double v, double *partials, unsigned slot) {
+  __shared__ double wsum[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) t += wsum[w];
+    partials[slot] = t;
+  }
+}
+
+// launchers defined in dslash.hip
+DslashArgs make_args(qcd_hip_lattice lat, double m, double tm);
+int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
+int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
+                  qcd_hip_field b, int rb, double m, double tm, double *partials, hipStream_t stream);
+
+}  // namespace qcdhip

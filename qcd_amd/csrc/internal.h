@@ -1,0 +1,105 @@
+// internal.h -- shared definitions of the MI355X (gfx950) lattice-QCD library.
+//
+// Device data layout (structure of arrays, one plane per complex component so
+// that 64 lanes working on 64 consecutive sites read one contiguous segment):
+//   spinor   comp = 3*spin + colour, 12 planes per right-hand side
+//            F64: double2 [nrhs][12][stride]
+//            F32: float2  [nrhs][12][stride]
+//            F16: __half2 [nrhs][12][stride] + float norm [nrhs][stride]
+//                 (value = half * norm, norm = max |real| over the site)
+//   gauge    link (mu, row, col): plane 9*mu + 3*row + col
+//            F64: double2 [36][stride]; F32: float2 [36][stride];
+//            F16: __half2 [36][stride] (SU(3) entries are bounded by 1)
+// Sites are ordered as on the host: l = t + T*(x + X*(y + Y*z)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qcd_hip.h"
+
+namespace qcdhip {
+
+constexpr int kBlock = 256;          // threads per workgroup for site kernels
+constexpr int kNumXcd = 8;           // MI355X: 8 accelerator complex dies
+constexpr int kMaxReduceBlocks = 1024;
+constexpr int kMaxKrylov = 64;       // largest GCR restart length
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define QH_CHECK(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) return ::qcdhip::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define QH_TRY(call)                 \
+  do {                               \
+    int rc_ = (call);                \
+    if (rc_ != 0) return rc_;        \
+  } while (0)
+
+// ---------------------------------------------------------------- context
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  size_t bytes_in_use = 0;            // device bytes held by library objects
+  // reduction scratch shared by the BLAS of this context
+  double *partials = nullptr;         // [kMaxReduceBlocks * 2 * (kMaxKrylov + 4)]
+  double *scalars = nullptr;          // small device scalar area
+  double *host_scalars = nullptr;     // pinned mirror for reads
+};
+
+Context *ctx();                       // current context (initialised on first use)
+int ensure_init();
+int dev_alloc(void **p, size_t bytes);
+void dev_free(void *p, size_t bytes);
+
+}  // namespace qcdhip
+
+// ---------------------------------------------------------------- objects
+struct qcd_hip_lattice_s {
+  int L[4];
+  long V;            // sites
+  long stride;       // plane stride in sites (V rounded up to 64)
+  double theta[4];
+  double ph_re[4];   // e^{+i theta_mu / L_mu}
+  double ph_im[4];
+  int twisted[4];    // theta_mu != 0
+};
+
+struct qcd_hip_field_s {
+  int kind;          // QCD_HIP_SPINOR or QCD_HIP_GAUGE
+  int prec;          // QCD_HIP_F64 / F32 / F16
+  int nrhs;          // spinor: number of right-hand sides
+  qcd_hip_lattice lat;
+  void *data;
+  float *norm;       // F16 spinors only
+  size_t bytes;      // total device bytes (data + norm)
+};
+
+namespace qcdhip {
+
+// --------------------------------------------------------- small helpers
+inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
+inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
+inline size_t spinor_comp_bytes(int prec) { return prec == QCD_HIP_F64 ? 16 : (prec == QCD_HIP_F32 ? 8 : 4); }
+
+// Hardware dispatches workgroup i to XCD i % 8.  Remap so each XCD owns one
+// contiguous range of logical blocks: neighbouring sites then share an L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblk) {
+  unsigned q = nblk / kNumXcd, r = nblk % kNumXcd;
+  unsigned xcd = bid % kNumXcd, local = bid / kNumXcd;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+}  // namespace qcdhip

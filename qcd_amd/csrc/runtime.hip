@@ -1,0 +1,641 @@
+// runtime.hip -- context, errors, lattices, fields, host<->device transfer,
+// precision conversion and the synthetic-input generator.
+#include "internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <thread>
+
+namespace qcdhip {
+
+static thread_local char g_err[1024] = "";
+static Context g_ctx;
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return 2;
+}
+
+Context *ctx() { return &g_ctx; }
+
+int ensure_init() {
+  if (g_ctx.stream) return 0;
+  return qcd_hip_init(-1) ? 3 : 0;
+}
+
+int dev_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("out of device memory: requested %zu bytes with %zu held by the library", bytes,
+              g_ctx.bytes_in_use);
+    *p = nullptr;
+    return 4;
+  }
+  g_ctx.bytes_in_use += bytes;
+  return 0;
+}
+
+void dev_free(void *p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  g_ctx.bytes_in_use -= bytes;
+}
+
+// Device staging buffer for AoS transfers, grown on demand.
+static void *g_stage = nullptr;
+static size_t g_stage_bytes = 0;
+
+static int stage(size_t bytes, void **out) {
+  if (bytes > g_stage_bytes) {
+    if (g_stage) QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+    dev_free(g_stage, g_stage_bytes);
+    g_stage = nullptr;
+    g_stage_bytes = 0;
+    QH_TRY(dev_alloc(&g_stage, bytes));
+    g_stage_bytes = bytes;
+  }
+  *out = g_stage;
+  return 0;
+}
+
+// ------------------------------------------------------------ scratch fields
+// Library-owned spinors reused across calls (resized when a larger one is
+// needed), so that entry points needing temporaries do not allocate per call.
+static qcd_hip_field g_scratch[8] = {nullptr};
+
+int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_field *out) {
+  if (slot < 0 || slot >= 8) {
+    set_error("scratch slot %d out of range", slot);
+    return 1;
+  }
+  qcd_hip_field f = g_scratch[slot];
+  if (f && (f->prec != prec || f->nrhs < nrhs || f->lat->stride != lat->stride)) {
+    qcd_hip_field_destroy(f);
+    g_scratch[slot] = f = nullptr;
+  }
+  if (!f) {
+    QH_TRY(qcd_hip_spinor_create(&f, lat, prec, nrhs));
+    g_scratch[slot] = f;
+  }
+  f->lat = lat;
+  *out = f;
+  return 0;
+}
+
+// ------------------------------------------------------------ layout kernels
+// AoS site-major (site, k) -> planes (k, site) with precision change.
+template <typename Dst>
+__device__ __forceinline__ Dst cvt2(double2 v);
+template <>
+__device__ __forceinline__ double2 cvt2<double2>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 cvt2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <>
+__device__ __forceinline__ __half2 cvt2<__half2>(double2 v) { return __floats2half2_rn((float)v.x, (float)v.y); }
+
+template <typename Dst, int NC>
+__global__ void __launch_bounds__(kBlock) k_aos_to_planes(const double2 *__restrict__ aos, Dst *__restrict__ planes,
+                                                          long V, long stride) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+#pragma unroll
+  for (int k = 0; k < NC; k++) planes[k * stride + s] = cvt2<Dst>(aos[s * NC + k]);
+}
+
+template <typename Src, int NC>
+__global__ void __launch_bounds__(kBlock) k_planes_to_aos(const Src *__restrict__ planes, double2 *__restrict__ aos,
+                                                          long V, long stride) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    Src v = planes[k * stride + s];
+    double2 o;
+    if constexpr (sizeof(Src) == 16) {
+      o = *(const double2 *)&v;
+    } else if constexpr (sizeof(Src) == 8) {
+      float2 f = *(const float2 *)&v;
+      o = make_double2(f.x, f.y);
+    } else {
+      float2 f = __half22float2(*(const __half2 *)&v);
+      o = make_double2(f.x, f.y);
+    }
+    aos[s * NC + k] = o;
+  }
+}
+
+// Half-precision spinor with a per-site scale: norm = max |real| of the site.
+__global__ void __launch_bounds__(kBlock) k_aos_to_half_spinor(const double2 *__restrict__ aos, __half2 *__restrict__ planes,
+                                                               float *__restrict__ norm, long V, long stride) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  double mx = 0.0;
+  double2 v[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    v[k] = aos[s * 12 + k];
+    mx = fmax(mx, fmax(fabs(v[k].x), fabs(v[k].y)));
+  }
+  float n = (float)mx;
+  float inv = n > 0.f ? 1.0f / n : 0.f;
+  norm[s] = n;
+#pragma unroll
+  for (int k = 0; k < 12; k++) planes[k * stride + s] = __floats2half2_rn((float)v[k].x * inv, (float)v[k].y * inv);
+}
+
+__global__ void __launch_bounds__(kBlock) k_half_spinor_to_aos(const __half2 *__restrict__ planes, const float *__restrict__ norm,
+                                                               double2 *__restrict__ aos, long V, long stride) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  float n = norm[s];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    float2 f = __half22float2(planes[k * stride + s]);
+    aos[s * 12 + k] = make_double2((double)f.x * n, (double)f.y * n);
+  }
+}
+
+// plane-to-plane precision conversion; F16 spinors carry the site norm.
+template <typename Src>
+__device__ __forceinline__ double2 load_as_double(const Src *p, long i, const float *norm, long s) {
+  if constexpr (sizeof(Src) == 16) {
+    return *(const double2 *)&p[i];
+  } else if constexpr (sizeof(Src) == 8) {
+    float2 f = *(const float2 *)&p[i];
+    return make_double2(f.x, f.y);
+  } else {
+    float2 f = __half22float2(*(const __half2 *)&p[i]);
+    float n = norm ? norm[s] : 1.0f;
+    return make_double2((double)f.x * n, (double)f.y * n);
+  }
+}
+
+template <typename Src, typename Dst, int NC>
+__global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict__ src, const float *__restrict__ snorm,
+                                                           Dst *__restrict__ dst, float *__restrict__ dnorm,
+                                                           long V, long stride) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  double2 v[NC];
+  double mx = 0.0;
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    v[k] = load_as_double(src, k * stride + s, snorm, s);
+    mx = fmax(mx, fmax(fabs(v[k].x), fabs(v[k].y)));
+  }
+  double inv = 1.0;
+  if (dnorm) {
+    dnorm[s] = (float)mx;
+    inv = mx > 0.0 ? 1.0 / (double)(float)mx : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) dst[k * stride + s] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
+}
+
+}  // namespace qcdhip
+
+using namespace qcdhip;
+
+// ================================================================ C ABI
+extern "C" {
+
+const char *qcd_hip_last_error(void) { return g_err; }
+
+int qcd_hip_init(int device) {
+  if (g_ctx.stream) {
+    if (device < 0 || device == g_ctx.device) return 0;
+    set_error("qcd_hip_init: already initialised on device %d", g_ctx.device);
+    return 1;
+  }
+  int n = 0;
+  QH_CHECK(hipGetDeviceCount(&n));
+  if (n <= 0) {
+    set_error("qcd_hip_init: no HIP device visible");
+    return 1;
+  }
+  if (device < 0) QH_CHECK(hipGetDevice(&device));
+  if (device >= n) {
+    set_error("qcd_hip_init: device %d out of range (%d devices)", device, n);
+    return 1;
+  }
+  QH_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  QH_CHECK(hipGetDeviceProperties(&prop, device));
+  g_ctx.device = device;
+  g_ctx.num_cu = prop.multiProcessorCount;
+  QH_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  size_t pbytes = sizeof(double) * kMaxReduceBlocks * 2 * (kMaxKrylov + 4);
+  QH_TRY(dev_alloc((void **)&g_ctx.partials, pbytes));
+  QH_TRY(dev_alloc((void **)&g_ctx.scalars, sizeof(double) * 4096));
+  QH_CHECK(hipHostMalloc((void **)&g_ctx.host_scalars, sizeof(double) * 4096, hipHostMallocDefault));
+  return 0;
+}
+
+int qcd_hip_device_name(char *buf, int len) {
+  QH_TRY(ensure_init());
+  hipDeviceProp_t prop;
+  QH_CHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+  snprintf(buf, (size_t)len, "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+           prop.totalGlobalMem / 1e9);
+  return 0;
+}
+
+int qcd_hip_synchronize(void) {
+  QH_TRY(ensure_init());
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return 0;
+}
+
+size_t qcd_hip_bytes_in_use(void) { return g_ctx.bytes_in_use; }
+
+// ------------------------------------------------------------ lattices
+int qcd_hip_lattice_create(qcd_hip_lattice *out, const int L[4], const double theta[4]) {
+  *out = nullptr;
+  for (int d = 0; d < 4; d++)
+    if (L[d] < 2) {
+      set_error("lattice extent L[%d] = %d must be at least 2", d, L[d]);
+      return 1;
+    }
+  long V = (long)L[0] * L[1] * L[2] * L[3];
+  if (V % 2) {
+    set_error("lattice volume must be even (lib/src/qcd_init.c:45-51)");
+    return 1;
+  }
+  auto *lat = new qcd_hip_lattice_s();
+  for (int d = 0; d < 4; d++) {
+    lat->L[d] = L[d];
+    lat->theta[d] = theta ? theta[d] : 0.0;
+    double a = lat->theta[d] / (double)L[d];
+    lat->twisted[d] = lat->theta[d] != 0.0;
+    lat->ph_re[d] = lat->twisted[d] ? cos(a) : 1.0;
+    lat->ph_im[d] = lat->twisted[d] ? sin(a) : 0.0;
+  }
+  lat->V = V;
+  lat->stride = round_up(V, 64);
+  *out = lat;
+  return 0;
+}
+
+void qcd_hip_lattice_destroy(qcd_hip_lattice lat) { delete lat; }
+
+long qcd_hip_lattice_volume(qcd_hip_lattice lat) { return lat ? lat->V : 0; }
+
+// ------------------------------------------------------------ fields
+static int field_alloc(qcd_hip_field *out, qcd_hip_lattice lat, int kind, int prec, int nrhs) {
+  *out = nullptr;
+  QH_TRY(ensure_init());
+  if (!lat) {
+    set_error("field create: null lattice");
+    return 1;
+  }
+  if (prec < QCD_HIP_F64 || prec > QCD_HIP_F16) {
+    set_error("field create: unknown precision %d", prec);
+    return 1;
+  }
+  if (kind == QCD_HIP_SPINOR && nrhs < 1) {
+    set_error("spinor create: nrhs must be >= 1 (got %d)", nrhs);
+    return 1;
+  }
+  auto *f = new qcd_hip_field_s();
+  f->kind = kind;
+  f->prec = prec;
+  f->nrhs = kind == QCD_HIP_SPINOR ? nrhs : 1;
+  f->lat = lat;
+  size_t ncomp = kind == QCD_HIP_SPINOR ? 12 * (size_t)nrhs : 36;
+  size_t data_bytes = ncomp * (size_t)lat->stride * spinor_comp_bytes(prec);
+  size_t norm_bytes = (kind == QCD_HIP_SPINOR && prec == QCD_HIP_F16) ? sizeof(float) * (size_t)nrhs * lat->stride : 0;
+  int rc = dev_alloc(&f->data, data_bytes + norm_bytes);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  f->norm = norm_bytes ? (float *)((char *)f->data + data_bytes) : nullptr;
+  f->bytes = data_bytes + norm_bytes;
+  hipError_t e = hipMemsetAsync(f->data, 0, f->bytes, g_ctx.stream);
+  if (e != hipSuccess) {
+    dev_free(f->data, f->bytes);
+    delete f;
+    return hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__);
+  }
+  *out = f;
+  return 0;
+}
+
+int qcd_hip_spinor_create(qcd_hip_field *f, qcd_hip_lattice lat, int prec, int nrhs) {
+  return field_alloc(f, lat, QCD_HIP_SPINOR, prec, nrhs);
+}
+
+int qcd_hip_gauge_create(qcd_hip_field *f, qcd_hip_lattice lat, int prec) {
+  return field_alloc(f, lat, QCD_HIP_GAUGE, prec, 1);
+}
+
+void qcd_hip_field_destroy(qcd_hip_field f) {
+  if (!f) return;
+  (void)hipStreamSynchronize(g_ctx.stream);
+  dev_free(f->data, f->bytes);
+  delete f;
+}
+
+static void *rhs_ptr(qcd_hip_field f, int rhs) {
+  return (char *)f->data + (size_t)rhs * 12 * f->lat->stride * spinor_comp_bytes(f->prec);
+}
+
+int qcd_hip_spinor_upload(qcd_hip_field f, int rhs, const double *aos) {
+  QH_TRY(ensure_init());
+  if (!f || f->kind != QCD_HIP_SPINOR || rhs < 0 || rhs >= f->nrhs || !aos) {
+    set_error("spinor upload: bad field, rhs or pointer");
+    return 1;
+  }
+  const long V = f->lat->V, st = f->lat->stride;
+  void *buf;
+  QH_TRY(stage((size_t)V * 192, &buf));
+  QH_CHECK(hipMemcpyAsync(buf, aos, (size_t)V * 192, hipMemcpyHostToDevice, g_ctx.stream));
+  const double2 *src = (const double2 *)buf;
+  if (f->prec == QCD_HIP_F64)
+    k_aos_to_planes<double2, 12><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (double2 *)rhs_ptr(f, rhs), V, st);
+  else if (f->prec == QCD_HIP_F32)
+    k_aos_to_planes<float2, 12><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (float2 *)rhs_ptr(f, rhs), V, st);
+  else
+    k_aos_to_half_spinor<<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (__half2 *)rhs_ptr(f, rhs),
+                                                                   f->norm + (size_t)rhs * st, V, st);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return 0;
+}
+
+int qcd_hip_spinor_download(qcd_hip_field f, int rhs, double *aos) {
+  QH_TRY(ensure_init());
+  if (!f || f->kind != QCD_HIP_SPINOR || rhs < 0 || rhs >= f->nrhs || !aos) {
+    set_error("spinor download: bad field, rhs or pointer");
+    return 1;
+  }
+  const long V = f->lat->V, st = f->lat->stride;
+  void *buf;
+  QH_TRY(stage((size_t)V * 192, &buf));
+  double2 *dst = (double2 *)buf;
+  if (f->prec == QCD_HIP_F64)
+    k_planes_to_aos<double2, 12><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const double2 *)rhs_ptr(f, rhs), dst, V, st);
+  else if (f->prec == QCD_HIP_F32)
+    k_planes_to_aos<float2, 12><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const float2 *)rhs_ptr(f, rhs), dst, V, st);
+  else
+    k_half_spinor_to_aos<<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const __half2 *)rhs_ptr(f, rhs),
+                                                                   f->norm + (size_t)rhs * st, dst, V, st);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipMemcpyAsync(aos, buf, (size_t)V * 192, hipMemcpyDeviceToHost, g_ctx.stream));
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return 0;
+}
+
+int qcd_hip_gauge_upload(qcd_hip_field f, const double *aos) {
+  QH_TRY(ensure_init());
+  if (!f || f->kind != QCD_HIP_GAUGE || !aos) {
+    set_error("gauge upload: bad field or pointer");
+    return 1;
+  }
+  const long V = f->lat->V, st = f->lat->stride;
+  void *buf;
+  QH_TRY(stage((size_t)V * 576, &buf));
+  QH_CHECK(hipMemcpyAsync(buf, aos, (size_t)V * 576, hipMemcpyHostToDevice, g_ctx.stream));
+  const double2 *src = (const double2 *)buf;
+  if (f->prec == QCD_HIP_F64)
+    k_aos_to_planes<double2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (double2 *)f->data, V, st);
+  else if (f->prec == QCD_HIP_F32)
+    k_aos_to_planes<float2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (float2 *)f->data, V, st);
+  else
+    k_aos_to_planes<__half2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (__half2 *)f->data, V, st);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return 0;
+}
+
+int qcd_hip_gauge_download(qcd_hip_field f, double *aos) {
+  QH_TRY(ensure_init());
+  if (!f || f->kind != QCD_HIP_GAUGE || !aos) {
+    set_error("gauge download: bad field or pointer");
+    return 1;
+  }
+  const long V = f->lat->V, st = f->lat->stride;
+  void *buf;
+  QH_TRY(stage((size_t)V * 576, &buf));
+  double2 *dst = (double2 *)buf;
+  if (f->prec == QCD_HIP_F64)
+    k_planes_to_aos<double2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const double2 *)f->data, dst, V, st);
+  else if (f->prec == QCD_HIP_F32)
+    k_planes_to_aos<float2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const float2 *)f->data, dst, V, st);
+  else
+    k_planes_to_aos<__half2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const __half2 *)f->data, dst, V, st);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipMemcpyAsync(aos, buf, (size_t)V * 576, hipMemcpyDeviceToHost, g_ctx.stream));
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));
+  return 0;
+}
+
+}  // extern "C"
+
+template <typename Src, typename Dst, int NC>
+static void launch_convert(const void *src, const float *sn, void *dst, float *dn, long V, long st) {
+  k_convert_planes<Src, Dst, NC><<<grid_for(V), kBlock, 0, g_ctx.stream>>>((const Src *)src, sn, (Dst *)dst, dn, V, st);
+}
+
+template <int NC>
+static int convert_planes(int sp, const void *src, const float *sn, int dp, void *dst, float *dn, long V, long st) {
+  if (sp == QCD_HIP_F64 && dp == QCD_HIP_F64) launch_convert<double2, double2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F64 && dp == QCD_HIP_F32) launch_convert<double2, float2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F64 && dp == QCD_HIP_F16) launch_convert<double2, __half2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F32 && dp == QCD_HIP_F64) launch_convert<float2, double2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F32 && dp == QCD_HIP_F32) launch_convert<float2, float2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F32 && dp == QCD_HIP_F16) launch_convert<float2, __half2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F16 && dp == QCD_HIP_F64) launch_convert<__half2, double2, NC>(src, sn, dst, dn, V, st);
+  else if (sp == QCD_HIP_F16 && dp == QCD_HIP_F32) launch_convert<__half2, float2, NC>(src, sn, dst, dn, V, st);
+  else launch_convert<__half2, __half2, NC>(src, sn, dst, dn, V, st);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" {
+
+int qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src) {
+  QH_TRY(ensure_init());
+  if (!dst || !src || dst->kind != src->kind || dst->lat->V != src->lat->V || dst->nrhs != src->nrhs) {
+    set_error("field convert: fields differ in kind, lattice or nrhs");
+    return 1;
+  }
+  const long V = src->lat->V, st = src->lat->stride;
+  if (src->kind == QCD_HIP_GAUGE) return convert_planes<36>(src->prec, src->data, nullptr, dst->prec, dst->data, nullptr, V, st);
+  for (int r = 0; r < src->nrhs; r++) {
+    const float *sn = src->norm ? src->norm + (size_t)r * st : nullptr;
+    float *dn = dst->norm ? dst->norm + (size_t)r * st : nullptr;
+    QH_TRY(convert_planes<12>(src->prec, rhs_ptr(src, r), sn, dst->prec, rhs_ptr(dst, r), dn, V, st));
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ================================================================ generator
+// Counter-based draws: draw k of stream s under seed is
+//   mix(key + k * 0x632BE59BD9B4E019), key = mix(seed) ^ s * 0xD1B54A32D192ED03,
+// mix = the splitmix64 finaliser; uniforms in (0,1) from the top 53 bits;
+// normals by Box-Muller.  Links U = reunitarize(exp(i eps H)),
+// H = (A + A^+)/2 made traceless with A_ij = N(0,1) + i N(0,1); exp by a
+// degree-16 Taylor polynomial of X / 2^s (|X/2^s|_inf <= 1/4) and s squarings;
+// reunitarisation by Gram-Schmidt of rows 0 and 1, row 2 = conj(row0 x row1).
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+inline double unit_draw(uint64_t key, uint64_t k) {
+  uint64_t r = mix64(key + k * 0x632BE59BD9B4E019ULL);
+  return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+void gaussians(double *out, int n, uint64_t seed, uint64_t stream) {
+  const uint64_t key = mix64(seed) ^ (stream * 0xD1B54A32D192ED03ULL);
+  for (int i = 0; i < n; i += 2) {
+    double u1 = unit_draw(key, (uint64_t)i), u2 = unit_draw(key, (uint64_t)i + 1);
+    double r = std::sqrt(-2.0 * std::log(u1));
+    out[i] = r * std::cos(6.283185307179586 * u2);
+    out[i + 1] = r * std::sin(6.283185307179586 * u2);
+  }
+}
+
+struct C {
+  double re, im;
+};
+inline C cm(C a, C b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline C ca(C a, C b) { return {a.re + b.re, a.im + b.im}; }
+
+void m3mul(C r[3][3], const C a[3][3], const C b[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      C s = {0.0, 0.0};
+      for (int k = 0; k < 3; k++) s = ca(s, cm(a[i][k], b[k][j]));
+      r[i][j] = s;
+    }
+}
+
+void su3_link(C U[3][3], const double g[18], double eps) {
+  C H[3][3], X[3][3], E[3][3], T[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      H[i][j].re = 0.5 * (g[2 * (3 * i + j)] + g[2 * (3 * j + i)]);
+      H[i][j].im = 0.5 * (g[2 * (3 * i + j) + 1] - g[2 * (3 * j + i) + 1]);
+    }
+  double tr = (H[0][0].re + H[1][1].re + H[2][2].re) / 3.0;
+  for (int i = 0; i < 3; i++) {
+    H[i][i].re -= tr;
+    H[i][i].im = 0.0;
+  }
+  double nrm = 0.0, scale = 1.0;
+  int s = 0;
+  for (int i = 0; i < 3; i++) {
+    double row = 0.0;
+    for (int j = 0; j < 3; j++) {
+      X[i][j].re = -eps * H[i][j].im;
+      X[i][j].im = eps * H[i][j].re;
+      row += std::fabs(X[i][j].re) + std::fabs(X[i][j].im);
+    }
+    nrm = std::max(nrm, row);
+  }
+  while (nrm * scale > 0.25) {
+    scale *= 0.5;
+    s++;
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      X[i][j].re *= scale;
+      X[i][j].im *= scale;
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) E[i][j] = {i == j ? 1.0 : 0.0, 0.0};
+  for (int k = 16; k >= 1; k--) {
+    m3mul(T, X, E);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        E[i][j].re = T[i][j].re / (double)k + (i == j ? 1.0 : 0.0);
+        E[i][j].im = T[i][j].im / (double)k;
+      }
+  }
+  for (int k = 0; k < s; k++) {
+    m3mul(T, E, E);
+    std::memcpy(E, T, sizeof(E));
+  }
+  double n0 = 0.0, n1 = 0.0;
+  for (int j = 0; j < 3; j++) n0 += E[0][j].re * E[0][j].re + E[0][j].im * E[0][j].im;
+  n0 = std::sqrt(n0);
+  for (int j = 0; j < 3; j++) U[0][j] = {E[0][j].re / n0, E[0][j].im / n0};
+  C p = {0.0, 0.0};
+  for (int j = 0; j < 3; j++) p = ca(p, cm({U[0][j].re, -U[0][j].im}, E[1][j]));
+  for (int j = 0; j < 3; j++) {
+    C q = cm(p, U[0][j]);
+    U[1][j] = {E[1][j].re - q.re, E[1][j].im - q.im};
+  }
+  for (int j = 0; j < 3; j++) n1 += U[1][j].re * U[1][j].re + U[1][j].im * U[1][j].im;
+  n1 = std::sqrt(n1);
+  for (int j = 0; j < 3; j++) U[1][j] = {U[1][j].re / n1, U[1][j].im / n1};
+  for (int j = 0; j < 3; j++) {
+    int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    C x = cm(U[0][j1], U[1][j2]), y = cm(U[0][j2], U[1][j1]);
+    U[2][j] = {x.re - y.re, -(x.im - y.im)};
+  }
+}
+
+template <typename F>
+void parallel_for(long n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  long chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    long b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=]() {
+      for (long i = b; i < e; i++) f(i);
+    });
+  }
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcd_hip_gauge_generate_host(double *aos, long V, double eps, uint64_t seed) {
+  if (!aos || V <= 0) {
+    qcdhip::set_error("gauge generate: bad pointer or volume");
+    return 1;
+  }
+  parallel_for(4 * V, [=](long i) {
+    double g[18];
+    gaussians(g, 18, seed, (uint64_t)i);
+    su3_link(reinterpret_cast<C(*)[3]>(aos + 18 * i), g, eps);
+  });
+  return 0;
+}
+
+int qcd_hip_spinor_gaussian_host(double *aos, long V, uint64_t seed) {
+  if (!aos || V <= 0) {
+    qcdhip::set_error("spinor generate: bad pointer or volume");
+    return 1;
+  }
+  parallel_for(V, [=](long i) { gaussians(aos + 24 * i, 24, seed ^ 0x5350494E4F52ULL, (uint64_t)i); });
+  return 0;
+}
+
+}  // extern "C"
