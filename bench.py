@@ -50,12 +50,14 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     x = Q.Spinor(lat, p).upload(psi)
     y = Q.Spinor(lat, p)
     m = 1 / (2 * 0.132) - 4
+    cfg = Q.dslash_tune(y, x, g)
     Q.synchronize()
     t0 = time.perf_counter()
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
     wall = time.perf_counter() - t0
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
-    return {"prec": prec, "L": list(L), "ms": ms, "GBps": gbps, "wall_s": wall}
+    return {"prec": prec, "L": list(L), "ms": ms, "GBps": gbps, "wall_s": wall,
+            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab"][cfg[0]], "min_waves": cfg[1]}}
 
 
 def cpu_baseline_leg(L, seconds=10.0):
@@ -107,12 +109,14 @@ def main():
     extra = {}
     for prec in ("f32", "f16"):
         r = dslash_leg(Q, L, prec, args.steps, args.warmup)
-        extra[prec] = {"ms": r["ms"], "GBps": r["GBps"], "frac": r["GBps"] / HBM_PEAK_GBPS}
+        extra[prec] = {"ms": r["ms"], "GBps": r["GBps"], "frac": r["GBps"] / HBM_PEAK_GBPS, "config": r["config"]}
         log(f"{prec} dslash 24^3x48: {r['ms']*1e3:.1f} us, {r['GBps']:.0f} GB/s")
     L5 = (96, 48, 48, 48)
     big = dslash_leg(Q, L5, "f64", max(10, args.steps // 10), max(2, args.warmup // 10))
     log(f"f64 dslash 48^3x96: {big['ms']*1e3:.1f} us, {big['GBps']:.0f} GB/s")
-    extra["f64_48c96"] = {"ms": big["ms"], "GBps": big["GBps"], "frac": big["GBps"] / HBM_PEAK_GBPS}
+    extra["f64_48c96"] = {"ms": big["ms"], "GBps": big["GBps"], "frac": big["GBps"] / HBM_PEAK_GBPS,
+                          "config": big["config"]}
+    extra["f64"] = {"config": head["config"]}
 
     cpu = None
     if not args.no_cpu:

@@ -77,6 +77,15 @@ int qcd_hip_dslash(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge,
 int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge,
                         double m, double tm, int op, int warmup, int iters, double *avg_ms);
 
+/* Autotuner: time every traversal (0 = one contiguous range per XCD,
+ * 1 = one y-slab per XCD swept along z) and occupancy (0 = compiler default,
+ * 3 or 4 minimum waves per SIMD) on these fields and keep the fastest for
+ * their (lattice extents, precisions).  Runs by itself on first use unless
+ * the environment sets QCD_HIP_AUTOTUNE=0.  All candidates give bit-identical
+ * results. */
+int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, int tuned[2], double *ms);
+int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ);
+
 /* ------------------------------------------------------------ smearing
  * niter iterations of 3D Gaussian smearing of every rhs of v on time-slice t
  * (t < 0: every time-slice), in place (lib/src/qcd_smearing.c:25-199).

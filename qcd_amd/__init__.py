@@ -95,6 +95,8 @@ def lib():
         "qcd_hip_spinor_gaussian_host": (i, [_dp, lg, ctypes.c_uint64]),
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
+        "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
+        "qcd_hip_dslash_set_config": (i, [_vp, _vp, i, i]),
         "qcd_hip_gauss_smear": (i, [_vp, _vp, d, i, i]),
         "qcd_hip_norm2": (i, [_vp, i, _dp]),
         "qcd_hip_dot": (i, [_vp, i, _vp, i, _dp]),
@@ -287,6 +289,18 @@ def dslash_time(out, inp, gauge, m, tm=0.0, op=OP_D, warmup=5, iters=20):
     _check(lib().qcd_hip_dslash_time(out.h, inp.h, gauge.h, float(m), float(tm), int(op), int(warmup), int(iters),
                                      _p(ms)), "qcd_hip_dslash_time")
     return float(ms[0])
+
+
+def dslash_tune(out, inp, gauge):
+    """Autotune the Dslash launch configuration; returns (map, occ, ms)."""
+    cfg = (ctypes.c_int * 2)()
+    ms = np.zeros(1)
+    _check(lib().qcd_hip_dslash_tune(out.h, inp.h, gauge.h, cfg, _p(ms)), "qcd_hip_dslash_tune")
+    return int(cfg[0]), int(cfg[1]), float(ms[0])
+
+
+def dslash_set_config(inp, gauge, map_, occ):
+    _check(lib().qcd_hip_dslash_set_config(inp.h, gauge.h, int(map_), int(occ)), "qcd_hip_dslash_set_config")
 
 
 def gauss_smear(v, gauge, alpha, t, niter):

@@ -129,6 +129,15 @@ struct GaugeIn {
 };
 
 // ------------------------------------------------------------ geometry
+// Site traversal: how workgroups map onto lattice sites.
+//  MAP_XCD   : logical block b covers sites [256 b, 256 b + 256); logical
+//              blocks are spread so each XCD owns one contiguous range.
+//  MAP_SLAB  : XCD k owns the y-slab [y0[k], y0[k+1]) and its workgroups sweep
+//              it in (t, x, y, z) order, so the z-neighbour of a site was
+//              touched T*X*(y1-y0) sites earlier by the same XCD (same L2)
+//              instead of T*X*Y sites earlier.
+enum { MAP_XCD = 0, MAP_SLAB = 1 };
+
 struct DslashArgs {
   unsigned V;
   unsigned L[4];
@@ -136,10 +145,14 @@ struct DslashArgs {
   double ph_re[4], ph_im[4];
   int twisted[4];
   double mp4, tm;
+  int map;
+  unsigned slab_y0[kNumXcd + 1];   // MAP_SLAB: y range of each XCD
+  unsigned slab_blocks;            // MAP_SLAB: workgroups per XCD
 };
 
 struct Site {
   unsigned s, t, x, y, z;
+  bool valid;
 };
 
 __device__ __forceinline__ Site site_coords(unsigned s, const DslashArgs &a) {
@@ -151,6 +164,39 @@ __device__ __forceinline__ Site site_coords(unsigned s, const DslashArgs &a) {
   st.x = r - r2 * a.L[1];
   st.z = r2 / a.L[2];
   st.y = r2 - st.z * a.L[2];
+  return st;
+}
+
+// The site handled by this lane under the traversal a.map.
+__device__ __forceinline__ Site site_of_lane(const DslashArgs &a) {
+  Site st;
+  if (a.map == MAP_SLAB) {
+    const unsigned k = blockIdx.x % kNumXcd, local = blockIdx.x / kNumXcd;
+    const unsigned y0 = a.slab_y0[k], ny = a.slab_y0[k + 1] - y0;
+    const unsigned P = a.L[0] * a.L[1];
+    const unsigned j = local * kBlock + threadIdx.x;
+    st.valid = ny > 0 && j < P * ny * a.L[3];
+    if (!st.valid) {
+      st.s = st.t = st.x = st.y = st.z = 0;
+      return st;
+    }
+    const unsigned tx = j % P, r = j / P;
+    const unsigned yy = r % ny;
+    st.z = r / ny;
+    st.y = y0 + yy;
+    st.x = tx / a.L[0];
+    st.t = tx - st.x * a.L[0];
+    st.s = tx + P * (st.y + a.L[2] * st.z);
+    return st;
+  }
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  st.valid = s < a.V;
+  if (!st.valid) {
+    st.s = st.t = st.x = st.y = st.z = 0;
+    return st;
+  }
+  st = site_coords(s, a);
+  st.valid = true;
   return st;
 }
 
@@ -307,7 +353,13 @@ __device__ __forceinline__ void block_reduce_store(double v, double *partials, u
 // launchers defined in dslash.hip
 DslashArgs make_args(qcd_hip_lattice lat, double m, double tm);
 int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
+// asynchronous single-rhs launch; *grid_out (if given) receives the number of
+// workgroups, i.e. of |out|^2 partials written when red is set
 int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
-                  qcd_hip_field b, int rb, double m, double tm, double *partials, hipStream_t stream);
+                  qcd_hip_field b, int rb, double m, double tm, double *partials, hipStream_t stream, int *grid_out);
+// autotune the configuration for (in, g) once (call outside graph capture)
+int dslash_prepare(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
+// number of workgroups dslash_launch will use for (in, g)
+int dslash_grid(qcd_hip_field in, qcd_hip_field g);
 
 }  // namespace qcdhip

@@ -148,3 +148,31 @@ def test_full_size_properties(gpu):
     ys = Q.Spinor(lat)
     Q.dslash(ys, xs, g, m, 0.0, Q.OP_D)
     assert rel_l2(ys.download(), 2 * Da - 3j * Db) <= 1e-14
+
+
+def test_launch_configs_bit_identical(gpu):
+    """Every traversal / occupancy candidate of the autotuner gives the same bits."""
+    Q = gpu
+    L = (16, 8, 8, 8)
+    m = 1 / (2 * 0.13) - 4
+    u, psi = _fields(L, 0.3, 71, 72)
+    lat = Q.Lattice(L)
+    for prec in ("f64", "f32", "f16"):
+        p = Q.PREC[prec]
+        g64 = Q.Gauge(lat).upload(u)
+        g = g64
+        if p != Q.F64:
+            g = Q.Gauge(lat, p)
+            Q.convert(g, g64)
+        x = Q.Spinor(lat, p).upload(psi)
+        y = Q.Spinor(lat, p)
+        outs = []
+        for map_ in (0, 1):
+            for occ in (0, 3, 4):
+                Q.dslash_set_config(x, g, map_, occ)
+                Q.dslash(y, x, g, m, 0.01, Q.OP_Q)
+                outs.append(y.download())
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        mp, occ, ms = Q.dslash_tune(y, x, g)
+        assert mp in (0, 1) and occ in (0, 3, 4) and ms > 0
