@@ -78,7 +78,9 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
                         double m, double tm, int op, int warmup, int iters, double *avg_ms);
 
 /* Autotuner: time every traversal (0 = one contiguous range per XCD,
- * 1 = one y-slab per XCD swept along z) and occupancy (0 = compiler default,
+ * 1 = one y-slab per XCD swept along z, 2 | (n << 8) = each lane marches n
+ * steps along z carrying the centre spinor and the backward z hop in
+ * registers) and occupancy (0 = compiler default,
  * 3 or 4 minimum waves per SIMD) on these fields and keep the fastest for
  * their (lattice extents, precisions).  Runs by itself on first use unless
  * the environment sets QCD_HIP_AUTOTUNE=0.  All candidates give bit-identical

@@ -17,9 +17,12 @@
 
 namespace qcdhip {
 
-// OCC: 0 = compiler's register allocation, W > 0 = at least W waves per SIMD.
+// OCC & 15: 0 = compiler's register allocation, W > 0 = at least W waves per
+// SIMD; OCC & 16: scheduling fence between hop directions.
+#define QH_WAVES(OCC) __attribute__((amdgpu_waves_per_eu(((OCC) & 15) > 0 ? ((OCC) & 15) : 1, ((OCC) & 15) > 0 ? ((OCC) & 15) : 8)))
+
 template <int PS, int PG, int EPI, bool RED, int OCC>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, OCC > 0 ? OCC : 8)))
+__global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, SpinorIn<PS> b,
          double *__restrict__ partials) {
   using R = typename Prec<PS>::R;
@@ -27,31 +30,43 @@ k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, Spinor
   const unsigned s = st.s;
   double red = 0.0;
   if (st.valid) {
+    // same summation order as k_dslash_zm, so every traversal gives the same
+    // bits: acc = -2 diag(psi) + z back + z forward + t, x, y hops; out = -acc/2
     Cx<R> acc[4][3];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) acc[i][c] = Cx<R>{R(0), R(0)};
-
-    hop_dir<0>(acc, a, st, in, U);
-    hop_dir<1>(acc, a, st, in, U);
-    hop_dir<2>(acc, a, st, in, U);
-    hop_dir<3>(acc, a, st, in, U);
-
-    Cx<R> psi[12];
-    in.load(s, psi);
-    Cx<R> o[12];
     const R mp4 = R(a.mp4), tm = R(a.tm);
+    const bool tw = a.twisted[3] != 0;
+    {
+      Cx<R> psi[12], u[9], g0[3], g1[3];
+      in.load(s, psi);
 #pragma unroll
-    for (int sp = 0; sp < 4; sp++) {
-      const R g = sp < 2 ? tm : -tm;
+      for (int sp = 0; sp < 4; sp++) {
+        const R gg = sp < 2 ? tm : -tm;
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        Cx<R> v = psi[3 * sp + c];
-        Cx<R> d{mp4 * v.re - g * v.im, mp4 * v.im + g * v.re};
-        o[3 * sp + c] = Cx<R>{d.re - R(0.5) * acc[sp][c].re, d.im - R(0.5) * acc[sp][c].im};
+        for (int c = 0; c < 3; c++) {
+          const Cx<R> v = psi[3 * sp + c];
+          acc[sp][c] = Cx<R>{R(-2) * (mp4 * v.re - gg * v.im), R(-2) * (mp4 * v.im + gg * v.re)};
+        }
       }
+      const unsigned sb = neighbour<3, -1>(st, a), sf = neighbour<3, +1>(st, a);
+      in.load(sb, psi);
+      U.load(3, sb, u);
+      hop_half<3, +1, true>(g0, g1, psi, u, tw, Cx<R>{(R)a.ph_re[3], -(R)a.ph_im[3]});
+      rebuild_add<3, +1>(acc, g0, g1);
+      in.load(sf, psi);
+      U.load(3, s, u);
+      hop_half<3, -1, false>(g0, g1, psi, u, tw, Cx<R>{(R)a.ph_re[3], (R)a.ph_im[3]});
+      rebuild_add<3, -1>(acc, g0, g1);
     }
+    constexpr bool F = (OCC & 16) != 0;
+    hop_dir<0, PS, PG, R, F>(acc, a, st, in, U);
+    hop_dir<1, PS, PG, R, F>(acc, a, st, in, U);
+    hop_dir<2, PS, PG, R, F>(acc, a, st, in, U);
+    if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+    Cx<R> o[12];
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) o[3 * sp + c] = Cx<R>{R(-0.5) * acc[sp][c].re, R(-0.5) * acc[sp][c].im};
     apply_epilogue<EPI>(o, b, s);
     if (RED) {
 #pragma unroll
@@ -62,12 +77,103 @@ k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, Spinor
   if (RED) block_reduce_store(red, partials, blockIdx.x);
 }
 
+// z-marching Dslash: lane = one (t, x, y) column, steps z0 .. z0+zlen-1.
+// Only the backward z hop U_z(x-z)^+ (1+g_z) psi(x-z) (a colour-multiplied
+// half spinor, computed one step earlier) is carried between steps; the
+// centre spinor is reloaded (it was this lane's forward z neighbour one step
+// before, so it comes from cache).  Each step starts with the local terms
+// (mass, both z hops) so the centre spinor dies before the t/x/y hops load.
+// acc holds -2 diag(psi) + sum of hops, so out = -1/2 acc.
+template <int PS, int PG, int EPI, bool RED, int OCC>
+__global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
+k_dslash_zm(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, SpinorIn<PS> b,
+            double *__restrict__ partials) {
+  using R = typename Prec<PS>::R;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned cb = lb % a.col_blocks, zc = lb / a.col_blocks;
+  const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
+  const unsigned col = cb * kBlock + threadIdx.x;
+  const unsigned z0 = zc * a.zlen;
+  const unsigned z1 = min(z0 + a.zlen, a.L[3]);
+  double red = 0.0;
+  if (col < P3 && z0 < a.L[3]) {
+    Site st;
+    st.valid = true;
+    st.x = col / a.L[0];
+    st.t = col - st.x * a.L[0];
+    st.y = st.x / a.L[1];
+    st.x -= st.y * a.L[1];
+    const bool tw = a.twisted[3] != 0;
+    const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
+    const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
+    const R mp4 = R(a.mp4), tm = R(a.tm);
+    Cx<R> cz0[3], cz1[3];
+    {
+      const unsigned zm = z0 == 0 ? a.L[3] - 1 : z0 - 1;
+      const unsigned sm = col + P3 * zm;
+      Cx<R> psi[12], u[9];
+      in.load(sm, psi);
+      U.load(3, sm, u);
+      hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+    }
+    for (unsigned z = z0; z < z1; z++) {
+      st.z = z;
+      st.s = col + P3 * z;
+      const unsigned s = st.s;
+      Cx<R> acc[4][3];
+      {
+        Cx<R> psi[12], u[9], g0[3], g1[3];
+        in.load(s, psi);
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+          const R gg = sp < 2 ? tm : -tm;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const Cx<R> v = psi[3 * sp + c];
+            acc[sp][c] = Cx<R>{R(-2) * (mp4 * v.re - gg * v.im), R(-2) * (mp4 * v.im + gg * v.re)};
+          }
+        }
+        rebuild_add<3, +1>(acc, cz0, cz1);
+        U.load(3, s, u);
+        // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
+        hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+        const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
+        in.load(sf, psi);
+        hop_half<3, -1, false>(g0, g1, psi, u, tw, php);
+        rebuild_add<3, -1>(acc, g0, g1);
+      }
+      constexpr bool F = (OCC & 16) != 0;
+      if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+      hop_dir<0, PS, PG, R, F>(acc, a, st, in, U);
+      hop_dir<1, PS, PG, R, F>(acc, a, st, in, U);
+      hop_dir<2, PS, PG, R, F>(acc, a, st, in, U);
+      if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+      Cx<R> o[12];
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[3 * sp + c] = Cx<R>{R(-0.5) * acc[sp][c].re, R(-0.5) * acc[sp][c].im};
+      apply_epilogue<EPI>(o, b, s);
+      if (RED) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+      }
+      out.store(s, o);
+    }
+  }
+  if (RED) block_reduce_store(red, partials, blockIdx.x);
+}
+
 template <int PS, int PG, int EPI, bool RED, int OCC>
 static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
-  k_dslash<PS, PG, EPI, RED, OCC><<<grid, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                               SpinorOut<PS>::of(out, ro), bi, partials);
+  if (a.map == MAP_ZMARCH)
+    k_dslash_zm<PS, PG, EPI, RED, OCC><<<grid, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
+                                                                    SpinorOut<PS>::of(out, ro), bi, partials);
+  else
+    k_dslash<PS, PG, EPI, RED, OCC><<<grid, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
+                                                                 SpinorOut<PS>::of(out, ro), bi, partials);
   QH_CHECK(hipGetLastError());
   return 0;
 }
@@ -75,11 +181,14 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
 template <int PS, int PG, int EPI, bool RED>
 static int launch_one(const DslashArgs &a, int occ, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
+#define QH_O(N) launch_occ<PS, PG, EPI, RED, N>(a, grid, out, ro, in, ri, g, b, rb, partials, stream)
   switch (occ) {
-    case 3: return launch_occ<PS, PG, EPI, RED, 3>(a, grid, out, ro, in, ri, g, b, rb, partials, stream);
-    case 4: return launch_occ<PS, PG, EPI, RED, 4>(a, grid, out, ro, in, ri, g, b, rb, partials, stream);
-    default: return launch_occ<PS, PG, EPI, RED, 0>(a, grid, out, ro, in, ri, g, b, rb, partials, stream);
+    case 2: return QH_O(2);
+    case 3: return QH_O(3);
+    case 16: return QH_O(16);
+    default: return QH_O(0);
   }
+#undef QH_O
 }
 
 template <int PS, int PG>
@@ -117,8 +226,18 @@ DslashArgs make_args(qcd_hip_lattice lat, double m, double tm) {
   return a;
 }
 
-// Fill the traversal fields of a; returns the grid size.
+// Fill the traversal fields of a; returns the grid size.  For MAP_ZMARCH the
+// low byte of `map` is 2 and the z steps per lane are map >> 8.
 int set_traversal(DslashArgs &a, int map) {
+  if ((map & 0xff) == MAP_ZMARCH) {
+    const unsigned zlen = std::max(1u, std::min((unsigned)(map >> 8), a.L[3]));
+    const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
+    a.map = MAP_ZMARCH;
+    a.zlen = zlen;
+    a.col_blocks = (P3 + kBlock - 1) / kBlock;
+    const unsigned nz = (a.L[3] + zlen - 1) / zlen;
+    return (int)(a.col_blocks * nz);
+  }
   if (map == MAP_SLAB && a.L[2] >= (unsigned)kNumXcd) {
     a.map = MAP_SLAB;
     const unsigned P = a.L[0] * a.L[1];
@@ -184,8 +303,15 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
   return 1;
 }
 
-static const int kMaps[] = {MAP_XCD, MAP_SLAB};
-static const int kOccs[] = {0, 3, 4};
+// untuned default: z-march of 6 steps per lane
+static const int kDefaultMap = MAP_ZMARCH | (6 << 8);
+// Candidates: z-march lengths (1 = one site per lane).  The per-site
+// traversals MAP_XCD / MAP_SLAB stay selectable but are never faster: their
+// straight-line hop sequence lets the compiler hoist every load and spill.
+static const int kMaps[] = {MAP_ZMARCH | (1 << 8), MAP_ZMARCH | (2 << 8), MAP_ZMARCH | (3 << 8),
+                           MAP_ZMARCH | (4 << 8), MAP_ZMARCH | (6 << 8), MAP_ZMARCH | (8 << 8),
+                           MAP_ZMARCH | (12 << 8), MAP_ZMARCH | (16 << 8)};
+static const int kOccs[] = {0, 2, 3, 16};
 
 int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force) {
   const TuneKey key = key_of(in, g);
@@ -209,6 +335,9 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
       float ms = 0.f;
       QH_CHECK(hipEventElapsedTime(&ms, e0, e1));
       ms /= reps;
+      if (getenv("QCD_HIP_TUNE_VERBOSE"))
+        fprintf(stderr, "tune L=%dx%dx%dx%d ps=%d pg=%d map=%d zlen=%d occ=%d: %.2f us\n", key.L[0], key.L[1], key.L[2],
+                key.L[3], key.ps, key.pg, map & 0xff, map >> 8, occ, ms * 1e3);
       if (ms < best.ms) best = TuneVal{map, occ, ms};
     }
   }
@@ -253,8 +382,8 @@ int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
     set_error("dslash: out must not alias in");
     return 1;
   }
-  if (in->lat->V >= (1L << 31)) {
-    set_error("dslash: lattice too large for 32-bit site indices");
+  if (in->lat->stride * 12L * 16L >= (1L << 32)) {
+    set_error("dslash: lattice too large for 32-bit offsets (V must be below 2^24)");
     return 1;
   }
   return 0;
@@ -265,7 +394,7 @@ int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
 int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
                   qcd_hip_field b, int rb, double m, double tm, double *partials, hipStream_t stream, int *grid_out) {
   const TuneVal *t = tune_lookup(key_of(in, g));
-  const int map = t ? t->map : MAP_SLAB, occ = t ? t->occ : 0;
+  const int map = t ? t->map : kDefaultMap, occ = t ? t->occ : 0;
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out);
 }
 
@@ -278,7 +407,7 @@ int dslash_prepare(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
 int dslash_grid(qcd_hip_field in, qcd_hip_field g) {
   const TuneVal *t = tune_lookup(key_of(in, g));
   DslashArgs a = make_args(in->lat, 0.0, 0.0);
-  return set_traversal(a, t ? t->map : MAP_SLAB);
+  return set_traversal(a, t ? t->map : kDefaultMap);
 }
 
 }  // namespace qcdhip
@@ -350,7 +479,8 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
 // occ = 0 default, 3 or 4 minimum waves per SIMD) for these fields' key.
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ) {
   QH_TRY(ensure_init());
-  if (!in || !gauge || (map != 0 && map != 1) || (occ != 0 && occ != 3 && occ != 4)) {
+  const int m0 = map & 0xff;
+  if (!in || !gauge || m0 < 0 || m0 > 2 || (m0 == 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
     set_error("dslash_set_config: bad arguments");
     return 1;
   }

@@ -52,25 +52,44 @@ __device__ __forceinline__ T from_cx(Cx<R> v) {
 }
 
 // ------------------------------------------------------------ accessors
+// Component base pointers travel as kernel arguments (scalar registers); a
+// lane addresses every component with one 32-bit byte offset, so loads take
+// the scalar-base + vector-offset form and need no per-lane 64-bit addresses.
+// Host code guarantees 12 * stride * sizeof(complex) < 2^32 (one rhs of a
+// spinor, one direction of a gauge field).
+template <typename T>
+__device__ __forceinline__ T ld_plane(const T *base, unsigned off) {
+  return *(const T *)((const char *)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void st_plane(T *base, unsigned off, T v) {
+  *(T *)((char *)base + off) = v;
+}
+
 template <int P>
 struct SpinorIn {
   using T = typename Prec<P>::T;
   using R = typename Prec<P>::R;
-  const T *__restrict__ p;
-  const float *__restrict__ norm;
-  size_t stride;
+  const T *p[12];        // component k of tile 0
+  const float *norm;
   static SpinorIn of(qcd_hip_field f, int rhs) {
     SpinorIn s;
-    s.stride = (size_t)f->lat->stride;
-    s.p = (const T *)f->data + (size_t)rhs * 12 * s.stride;
-    s.norm = f->norm ? f->norm + (size_t)rhs * s.stride : nullptr;
+    const size_t st = (size_t)f->lat->stride;
+    const T *b = (const T *)f->data + (size_t)rhs * 12 * st;
+    for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
+    s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
     return s;
   }
+  // byte offset of site s from its tile-0 component pointer
+  static __device__ __forceinline__ unsigned offset(unsigned s) {
+    return ((s / kTile) * (12u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+  }
   __device__ __forceinline__ void load(unsigned s, Cx<R> v[12]) const {
+    const unsigned off = offset(s);
 #pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(p[k * stride + s]);
+    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(p[k], off));
     if constexpr (P == QCD_HIP_F16) {
-      const R n = norm[s];
+      const R n = ld_plane(norm, s * 4u);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         v[k].re *= n;
@@ -84,28 +103,29 @@ template <int P>
 struct SpinorOut {
   using T = typename Prec<P>::T;
   using R = typename Prec<P>::R;
-  T *__restrict__ p;
-  float *__restrict__ norm;
-  size_t stride;
+  T *p[12];
+  float *norm;
   static SpinorOut of(qcd_hip_field f, int rhs) {
     SpinorOut s;
-    s.stride = (size_t)f->lat->stride;
-    s.p = (T *)f->data + (size_t)rhs * 12 * s.stride;
-    s.norm = f->norm ? f->norm + (size_t)rhs * s.stride : nullptr;
+    const size_t st = (size_t)f->lat->stride;
+    T *b = (T *)f->data + (size_t)rhs * 12 * st;
+    for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
+    s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
     return s;
   }
   __device__ __forceinline__ void store(unsigned s, const Cx<R> v[12]) const {
+    const unsigned off = SpinorIn<P>::offset(s);
     if constexpr (P == QCD_HIP_F16) {
       R mx = R(0);
 #pragma unroll
       for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
       const R inv = mx > R(0) ? R(1) / mx : R(0);
-      norm[s] = mx;
+      st_plane(norm, s * 4u, (float)mx);
 #pragma unroll
-      for (int k = 0; k < 12; k++) p[k * stride + s] = from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv});
+      for (int k = 0; k < 12; k++) st_plane(p[k], off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) p[k * stride + s] = from_cx<T>(v[k]);
+      for (int k = 0; k < 12; k++) st_plane(p[k], off, from_cx<T>(v[k]));
     }
   }
 };
@@ -113,18 +133,19 @@ struct SpinorOut {
 template <int P>
 struct GaugeIn {
   using T = typename Prec<P>::T;
-  const T *__restrict__ p;
-  size_t stride;
+  const T *p[36];        // link component k of direction mu, tile 0, at p[9 mu + k]
   static GaugeIn of(qcd_hip_field f) {
     GaugeIn g;
-    g.stride = (size_t)f->lat->stride;
-    g.p = (const T *)f->data;
+    const size_t st = (size_t)f->lat->stride;
+    for (int mu = 0; mu < 4; mu++)
+      for (int k = 0; k < 9; k++) g.p[9 * mu + k] = (const T *)f->data + (size_t)mu * 9 * st + k * kTile;
     return g;
   }
   template <typename R>
   __device__ __forceinline__ void load(int mu, unsigned s, Cx<R> u[9]) const {
+    const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = to_cx<R>(p[(size_t)(9 * mu + k) * stride + s]);
+    for (int k = 0; k < 9; k++) u[k] = to_cx<R>(ld_plane(p[9 * mu + k], off));
   }
 };
 
@@ -136,7 +157,10 @@ struct GaugeIn {
 //              it in (t, x, y, z) order, so the z-neighbour of a site was
 //              touched T*X*(y1-y0) sites earlier by the same XCD (same L2)
 //              instead of T*X*Y sites earlier.
-enum { MAP_XCD = 0, MAP_SLAB = 1 };
+//  MAP_ZMARCH: each lane owns one (t, x, y) column and steps through zlen
+//              consecutive z values, carrying the centre spinor and the
+//              backward z hop in registers (2.5D blocking).
+enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2 };
 
 struct DslashArgs {
   unsigned V;
@@ -148,6 +172,8 @@ struct DslashArgs {
   int map;
   unsigned slab_y0[kNumXcd + 1];   // MAP_SLAB: y range of each XCD
   unsigned slab_blocks;            // MAP_SLAB: workgroups per XCD
+  unsigned zlen;                   // MAP_ZMARCH: z steps per lane
+  unsigned col_blocks;             // MAP_ZMARCH: workgroups per z chunk
 };
 
 struct Site {
@@ -274,13 +300,52 @@ __device__ __forceinline__ void hop_accumulate(Cx<R> acc[4][3], const Cx<R> psi[
   }
   su3_mul<DAG>(g0, u, h0);
   su3_mul<DAG>(g1, u, h1);
-  if (twisted) {
+  // Unconditional: an untwisted direction carries the phase (1, 0), which
+  // multiplies exactly and avoids a divergent-looking branch that doubles
+  // the live registers.
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
-      g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
-    }
+  for (int c = 0; c < 3; c++) {
+    g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
+    g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
   }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    acc[0][c] = cadd(acc[0][c], g0[c]);
+    acc[1][c] = cadd(acc[1][c], g1[c]);
+    acc[2][c] = cadd(acc[2][c], ipow<PJ::C + sp>(PJ::R == 0 ? g0[c] : g1[c]));
+    acc[3][c] = cadd(acc[3][c], ipow<PJ::D + sp>(PJ::W == 0 ? g0[c] : g1[c]));
+  }
+}
+
+// the colour-multiplied, phased half spinor g of one hop (before rebuilding)
+template <int MU, int S, bool DAG, typename R>
+__device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> psi[12], const Cx<R> u[9], bool twisted,
+                                         Cx<R> ph) {
+  using PJ = Proj<MU>;
+  constexpr int sp = S > 0 ? 0 : 2;
+  Cx<R> h0[3], h1[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h0[c] = cadd(psi[c], ipow<PJ::A + sp>(psi[3 * PJ::P + c]));
+    h1[c] = cadd(psi[3 + c], ipow<PJ::B + sp>(psi[3 * PJ::Q + c]));
+  }
+  su3_mul<DAG>(g0, u, h0);
+  su3_mul<DAG>(g1, u, h1);
+  // Unconditional: an untwisted direction carries the phase (1, 0), which
+  // multiplies exactly and avoids a divergent-looking branch that doubles
+  // the live registers.
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
+    g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
+  }
+}
+
+// add a half spinor of (1 + S g_MU) rebuilt to four spins
+template <int MU, int S, typename R>
+__device__ __forceinline__ void rebuild_add(Cx<R> acc[4][3], const Cx<R> g0[3], const Cx<R> g1[3]) {
+  using PJ = Proj<MU>;
+  constexpr int sp = S > 0 ? 0 : 2;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     acc[0][c] = cadd(acc[0][c], g0[c]);
@@ -292,9 +357,12 @@ __device__ __forceinline__ void hop_accumulate(Cx<R> acc[4][3], const Cx<R> psi[
 
 // forward hop U_mu(x)(1-g_mu)psi(x+mu) e^{+i th/L} and backward hop
 // U_mu(x-mu)^+ (1+g_mu) psi(x-mu) e^{-i th/L}
-template <int MU, int PS, int PG, typename R>
+// FENCE: keep the loads of this direction from being hoisted above the
+// previous one (bounds register pressure at the cost of memory parallelism).
+template <int MU, int PS, int PG, typename R, bool FENCE = false>
 __device__ __forceinline__ void hop_dir(Cx<R> acc[4][3], const DslashArgs &a, const Site &st, const SpinorIn<PS> &in,
                                         const GaugeIn<PG> &U) {
+  if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
   const bool tw = a.twisted[MU] != 0;
   const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
   const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};

@@ -1,16 +1,15 @@
 // internal.h -- shared definitions of the MI355X (gfx950) lattice-QCD library.
 //
-// Device data layout (structure of arrays, one plane per complex component so
-// that 64 lanes working on 64 consecutive sites read one contiguous segment):
-//   spinor   comp = 3*spin + colour, 12 planes per right-hand side
-//            F64: double2 [nrhs][12][stride]
-//            F32: float2  [nrhs][12][stride]
-//            F16: __half2 [nrhs][12][stride] + float norm [nrhs][stride]
-//                 (value = half * norm, norm = max |real| over the site)
-//   gauge    link (mu, row, col): plane 9*mu + 3*row + col
-//            F64: double2 [36][stride]; F32: float2 [36][stride];
-//            F16: __half2 [36][stride] (SU(3) entries are bounded by 1)
-// Sites are ordered as on the host: l = t + T*(x + X*(y + Y*z)).
+// Device data layout: tiles of 64 sites (one wavefront), complex components
+// of a tile stored one after another (array of structures of arrays).  A
+// wave reading component k of its 64 sites reads one contiguous segment, and
+// all components of a tile sit in one small region (DRAM page locality).
+//   spinor   comp k = 3*spin + colour; per right-hand side
+//            [stride/64][12][64] complex; F64 double2, F32 float2, F16 __half2
+//            + float norm [nrhs][stride] (value = half * norm, norm = max |re,im|)
+//   gauge    one array per direction mu: [4][stride/64][9][64] complex with
+//            k = 3*row + col (F16 needs no norm: SU(3) entries are bounded by 1)
+// Sites are numbered as on the host: l = t + T*(x + X*(y + Y*z)).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -94,6 +93,16 @@ namespace qcdhip {
 inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
 inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 inline size_t spinor_comp_bytes(int prec) { return prec == QCD_HIP_F64 ? 16 : (prec == QCD_HIP_F32 ? 8 : 4); }
+
+constexpr int kTile = 64;   // sites per layout tile
+// element index of component k (of nc) of site s inside one tiled array
+__host__ __device__ __forceinline__ size_t tile_index(int nc, int k, size_t s) {
+  return ((s / kTile) * (size_t)nc + (size_t)k) * kTile + (s % kTile);
+}
+// element index of gauge component k = 9*mu + 3*row + col of site s
+__host__ __device__ __forceinline__ size_t gauge_index(int k, size_t s, size_t stride) {
+  return (size_t)(k / 9) * 9 * stride + tile_index(9, k % 9, s);
+}
 
 // Hardware dispatches workgroup i to XCD i % 8.  Remap so each XCD owns one
 // contiguous range of logical blocks: neighbouring sites then share an L2.

@@ -94,6 +94,12 @@ int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_fi
 }
 
 // ------------------------------------------------------------ layout kernels
+// index of component k of site s: NC = 12 spinor, NC = 36 gauge field
+__device__ __forceinline__ size_t elem(int nc, int k, long s, long stride) {
+  return nc == 36 ? gauge_index(k, (size_t)s, (size_t)stride) : tile_index(nc, k, (size_t)s);
+}
+
+
 // AoS site-major (site, k) -> planes (k, site) with precision change.
 template <typename Dst>
 __device__ __forceinline__ Dst cvt2(double2 v);
@@ -110,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_planes(const double2 *__restr
   long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= V) return;
 #pragma unroll
-  for (int k = 0; k < NC; k++) planes[k * stride + s] = cvt2<Dst>(aos[s * NC + k]);
+  for (int k = 0; k < NC; k++) planes[elem(NC, k, s, stride)] = cvt2<Dst>(aos[s * NC + k]);
 }
 
 template <typename Src, int NC>
@@ -120,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_planes_to_aos(const Src *__restrict_
   if (s >= V) return;
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    Src v = planes[k * stride + s];
+    Src v = planes[elem(NC, k, s, stride)];
     double2 o;
     if constexpr (sizeof(Src) == 16) {
       o = *(const double2 *)&v;
@@ -151,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_half_spinor(const double2 *__
   float inv = n > 0.f ? 1.0f / n : 0.f;
   norm[s] = n;
 #pragma unroll
-  for (int k = 0; k < 12; k++) planes[k * stride + s] = __floats2half2_rn((float)v[k].x * inv, (float)v[k].y * inv);
+  for (int k = 0; k < 12; k++) planes[tile_index(12, k, s)] = __floats2half2_rn((float)v[k].x * inv, (float)v[k].y * inv);
 }
 
 __global__ void __launch_bounds__(kBlock) k_half_spinor_to_aos(const __half2 *__restrict__ planes, const float *__restrict__ norm,
@@ -161,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_half_spinor_to_aos(const __half2 *__
   float n = norm[s];
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    float2 f = __half22float2(planes[k * stride + s]);
+    float2 f = __half22float2(planes[tile_index(12, k, s)]);
     aos[s * 12 + k] = make_double2((double)f.x * n, (double)f.y * n);
   }
 }
@@ -191,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict
   double mx = 0.0;
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    v[k] = load_as_double(src, k * stride + s, snorm, s);
+    v[k] = load_as_double(src, elem(NC, k, s, stride), snorm, s);
     mx = fmax(mx, fmax(fabs(v[k].x), fabs(v[k].y)));
   }
   double inv = 1.0;
@@ -200,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict
     inv = mx > 0.0 ? 1.0 / (double)(float)mx : 0.0;
   }
 #pragma unroll
-  for (int k = 0; k < NC; k++) dst[k * stride + s] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
+  for (int k = 0; k < NC; k++) dst[elem(NC, k, s, stride)] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
 }
 
 }  // namespace qcdhip
@@ -268,6 +274,10 @@ int qcd_hip_lattice_create(qcd_hip_lattice *out, const int L[4], const double th
       return 1;
     }
   long V = (long)L[0] * L[1] * L[2] * L[3];
+  if (V >= (1L << 24)) {
+    set_error("lattice of %ld sites exceeds the 2^24-site limit of 32-bit in-field offsets", V);
+    return 1;
+  }
   if (V % 2) {
     set_error("lattice volume must be even (lib/src/qcd_init.c:45-51)");
     return 1;

@@ -6,6 +6,7 @@ Runs `iters` Wilson Dslash applications and `iters` plain FP64 copies
 the same lattice, so that the counters of the copy calibrate those of Dslash.
 """
 import argparse
+import os
 import sys
 
 import numpy as np
@@ -19,6 +20,10 @@ def main(argv=None):
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16"])
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--op", default="D", choices=["D", "Q"])
+    ap.add_argument("--map", type=int, default=None,
+                    help="force a traversal (0 XCD ranges, 1 y-slabs, 2 + 256*n z-march of n steps)")
+    ap.add_argument("--occ", type=int, default=0, help="occupancy variant when --map is given")
+    ap.add_argument("--sweep", action="store_true", help="time every candidate (QCD_HIP_TUNE_VERBOSE)")
     args = ap.parse_args(argv)
     L = tuple(args.lattice)
     Q.init(0)
@@ -34,6 +39,11 @@ def main(argv=None):
     x = Q.Spinor(lat, p).upload(psi)
     y = Q.Spinor(lat, p)
     m = 1 / (2 * 0.132) - 4
+    if args.sweep:
+        os.environ["QCD_HIP_TUNE_VERBOSE"] = "1"
+        print("tuned:", Q.dslash_tune(y, x, g), flush=True)
+    if args.map is not None:
+        Q.dslash_set_config(x, g, args.map, args.occ)
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_Q if args.op == "Q" else Q.OP_D, warmup=0, iters=args.iters)
     a = Q.Spinor(lat).upload(psi)
     b = Q.Spinor(lat)

@@ -151,13 +151,15 @@ def test_full_size_properties(gpu):
 
 
 def test_launch_configs_bit_identical(gpu):
-    """Every traversal / occupancy candidate of the autotuner gives the same bits."""
+    """Autotuner candidates: FP64 gives the same bits for every traversal and
+    occupancy; FP32/FP16 agree to rounding (the compiler's FMA contraction
+    differs between code shapes)."""
     Q = gpu
     L = (16, 8, 8, 8)
     m = 1 / (2 * 0.13) - 4
     u, psi = _fields(L, 0.3, 71, 72)
     lat = Q.Lattice(L)
-    for prec in ("f64", "f32", "f16"):
+    for prec, tol in (("f64", 0.0), ("f32", 1e-6), ("f16", 1e-3)):
         p = Q.PREC[prec]
         g64 = Q.Gauge(lat).upload(u)
         g = g64
@@ -166,13 +168,18 @@ def test_launch_configs_bit_identical(gpu):
             Q.convert(g, g64)
         x = Q.Spinor(lat, p).upload(psi)
         y = Q.Spinor(lat, p)
-        outs = []
-        for map_ in (0, 1):
-            for occ in (0, 3, 4):
+        by_kernel = {}
+        for map_ in (0, 1, 2 | (1 << 8), 2 | (3 << 8), 2 | (5 << 8)):
+            for occ in (0, 2, 3, 16):
                 Q.dslash_set_config(x, g, map_, occ)
                 Q.dslash(y, x, g, m, 0.01, Q.OP_Q)
-                outs.append(y.download())
-        for o in outs[1:]:
-            assert np.array_equal(o, outs[0])
+                by_kernel.setdefault(map_ & 0xff == 2, []).append(y.download())
+        ref = by_kernel[True][0]
+        for outs in by_kernel.values():
+            for o in outs:
+                if tol == 0.0:
+                    assert np.array_equal(o, ref)
+                else:
+                    assert rel_l2(o, ref) <= tol
         mp, occ, ms = Q.dslash_tune(y, x, g)
-        assert mp in (0, 1) and occ in (0, 3, 4) and ms > 0
+        assert (mp & 0xff) in (0, 1, 2) and occ in (0, 2, 3, 16) and ms > 0
