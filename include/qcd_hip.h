@@ -106,6 +106,46 @@ int qcd_hip_axpy(qcd_hip_field y, int ry, double are, double aim, qcd_hip_field 
 int qcd_hip_lincomb(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx,
                     double bre, double bim, qcd_hip_field y, int ry, int t0);
 
+/* ------------------------------------------------------------ solver
+ * Propagator solve of invert.c on the GPU: an FP64 outer loop on the true
+ * residual (invert.c:255-294) around GCR(n_restart) (invert.c:20-113) run in
+ * the inner precision: QCD_HIP_F64 is the plain solver, F32 / F16 (storage,
+ * FP32 arithmetic) give mixed-precision defect correction.  Solves D x = b
+ * for every rhs of b (with op = Q it solves Q x = i g5 b, the same system, as
+ * invert.c does) until ||b - D x|| / ||b|| <= tol, checked in FP64. */
+typedef struct qcd_hip_solver_s *qcd_hip_solver;
+typedef struct {
+   int    n_restart;   /* GCR steps per cycle (invert.c <N_restart>), 1..64 */
+   int    max_outer;   /* outer cycles (invert.c maxIter) */
+   double tol;         /* target relative residual (invert.c maxRes) */
+   int    inner_prec;  /* QCD_HIP_F64, QCD_HIP_F32 or QCD_HIP_F16 */
+   int    op;          /* QCD_HIP_OP_Q (as invert.c) or QCD_HIP_OP_D */
+   int    keep_q1;     /* 1: drop the first GCR direction from the cycle's
+                          solution exactly as invert.c:50-53 does */
+} qcd_hip_solver_params;
+typedef struct {
+   int    outer;        /* outer cycles (summed over rhs) */
+   int    inner_iters;  /* GCR iterations (summed over rhs) */
+   double relres;       /* largest final ||b - D x|| / ||b|| over rhs (FP64) */
+   double seconds;      /* wall time of the call */
+} qcd_hip_solver_stats;
+
+void qcd_hip_solver_default_params(qcd_hip_solver_params *p);
+int  qcd_hip_solver_create(qcd_hip_solver *s, qcd_hip_lattice lat, const qcd_hip_solver_params *p);
+void qcd_hip_solver_destroy(qcd_hip_solver s);
+/* set (or refresh after re-upload) the FP64 gauge field; builds the inner-
+ * precision copy */
+int  qcd_hip_solver_set_gauge(qcd_hip_solver s, qcd_hip_field gauge64);
+/* x, b: FP64 spinors with equal rhs counts; x is overwritten.  Returns 0 when
+ * every rhs converged, 5 on Krylov breakdown, 6 when max_outer ran out. */
+int  qcd_hip_solver_solve(qcd_hip_solver s, qcd_hip_field x, qcd_hip_field b, double kappa, double mu,
+                          qcd_hip_solver_stats *stats);
+int  qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field gauge64, double kappa, double mu,
+                    const qcd_hip_solver_params *p, qcd_hip_solver_stats *stats);
+/* ||D x[rx] - b[rb]|| / ||b[rb]|| in FP64, m = 1/(2 kappa) - 4 (b_minus_Dx.c:152-167) */
+int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
+                        double kappa, double mu, double *rel);
+
 #ifdef __cplusplus
 }
 #endif

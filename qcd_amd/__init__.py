@@ -63,6 +63,18 @@ class QcdComplex(ctypes.Structure):
     _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
 
 
+class SolverParams(ctypes.Structure):
+    """qcd_hip_solver_params (include/qcd_hip.h)."""
+    _fields_ = [("n_restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("tol", ctypes.c_double),
+                ("inner_prec", ctypes.c_int), ("op", ctypes.c_int), ("keep_q1", ctypes.c_int)]
+
+
+class SolverStats(ctypes.Structure):
+    """qcd_hip_solver_stats (include/qcd_hip.h)."""
+    _fields_ = [("outer", ctypes.c_int), ("inner_iters", ctypes.c_int), ("relres", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
 def lib():
     """Load libqcd_hip.so (raises QcdError when it has not been built)."""
     global _lib
@@ -102,6 +114,13 @@ def lib():
         "qcd_hip_dot": (i, [_vp, i, _vp, i, _dp]),
         "qcd_hip_axpy": (i, [_vp, i, d, d, _vp, i]),
         "qcd_hip_lincomb": (i, [_vp, i, d, d, _vp, i, d, d, _vp, i, i]),
+        "qcd_hip_solver_default_params": (None, [ctypes.POINTER(SolverParams)]),
+        "qcd_hip_solver_create": (i, [ctypes.POINTER(_vp), _vp, ctypes.POINTER(SolverParams)]),
+        "qcd_hip_solver_destroy": (None, [_vp]),
+        "qcd_hip_solver_set_gauge": (i, [_vp, _vp]),
+        "qcd_hip_solver_solve": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverStats)]),
+        "qcd_hip_invert": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverParams), ctypes.POINTER(SolverStats)]),
+        "qcd_hip_b_minus_Dx": (i, [_vp, i, _vp, i, _vp, d, d, _dp]),
         # drop-in reference API
         "qcd_initGeometry": (i, [pg, ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_uint16), _dp, i, i]),
         "qcd_destroyGeometry": (None, [pg]),
@@ -321,6 +340,60 @@ def lincomb(out, a, x, b=0.0, y=None, ro=0, rx=0, ry=0, t0=-1):
     _check(lib().qcd_hip_lincomb(out.h, int(ro), float(np.real(a)), float(np.imag(a)), x.h, int(rx),
                                  float(np.real(b)), float(np.imag(b)), y.h if y is not None else None, int(ry),
                                  int(t0)), "qcd_hip_lincomb")
+
+
+# ------------------------------------------------------------ solver
+def solver_params(**kw):
+    """Default qcd_hip_solver_params with keyword overrides (inner_prec may be 'f64'/'f32'/'f16')."""
+    p = SolverParams()
+    lib().qcd_hip_solver_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "inner_prec" and isinstance(v, str):
+            v = PREC[v]
+        if k == "op" and isinstance(v, str):
+            v = OP_Q if v == "Q" else OP_D
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """Resident solver workspace (invert.c's GCR + FP64 outer refinement)."""
+
+    def __init__(self, lat, gauge64, **kw):
+        self.lat = lat
+        self.params = solver_params(**kw)
+        h = _vp()
+        _check(lib().qcd_hip_solver_create(ctypes.byref(h), lat.h, ctypes.byref(self.params)),
+               "qcd_hip_solver_create")
+        self.h = h
+        self.set_gauge(gauge64)
+
+    def set_gauge(self, gauge64):
+        self.gauge = gauge64
+        _check(lib().qcd_hip_solver_set_gauge(self.h, gauge64.h), "qcd_hip_solver_set_gauge")
+
+    def solve(self, x, b, kappa, mu=0.0, check=True):
+        st = SolverStats()
+        rc = lib().qcd_hip_solver_solve(self.h, x.h, b.h, float(kappa), float(mu), ctypes.byref(st))
+        if check:
+            _check(rc, "qcd_hip_solver_solve")
+        return {"rc": rc, "outer": st.outer, "inner_iters": st.inner_iters, "relres": st.relres,
+                "seconds": st.seconds}
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qcd_hip_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def b_minus_Dx(x, b, gauge64, kappa, mu=0.0, rx=0, rb=0):
+    rel = np.zeros(1)
+    _check(lib().qcd_hip_b_minus_Dx(x.h, int(rx), b.h, int(rb), gauge64.h, float(kappa), float(mu), _p(rel)),
+           "qcd_hip_b_minus_Dx")
+    return float(rel[0])
 
 
 # ------------------------------------------------------------ drop-in helpers

@@ -5,6 +5,59 @@
 
 namespace qcdhip {
 
+// A set of spinors stored back to back (the rhs of one field), addressed by
+// index: the kernels of the solver take the whole Krylov basis this way.
+template <int P>
+struct SpinorSet {
+  using T = typename Prec<P>::T;
+  using R = typename Prec<P>::R;
+  T *base;            // component 0, tile 0 of vector 0
+  float *norm;        // F16 only
+  size_t vstride;     // elements between vectors
+  size_t nstride;     // norms between vectors
+  static SpinorSet of(qcd_hip_field f) {
+    SpinorSet s;
+    s.base = (T *)f->data;
+    s.norm = f->norm;
+    s.vstride = 12 * (size_t)f->lat->stride;
+    s.nstride = (size_t)f->lat->stride;
+    return s;
+  }
+  __device__ __forceinline__ void load(int j, unsigned s, Cx<R> v[12]) const {
+    const T *b = base + (size_t)j * vstride;
+    const unsigned off = SpinorIn<P>::offset(s);
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
+    if constexpr (P == QCD_HIP_F16) {
+      const R n = norm[(size_t)j * nstride + s];
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        v[k].re *= n;
+        v[k].im *= n;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(int j, unsigned s, const Cx<R> v[12]) const {
+    T *b = base + (size_t)j * vstride;
+    const unsigned off = SpinorIn<P>::offset(s);
+    if constexpr (P == QCD_HIP_F16) {
+      R mx = R(0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+      // a site whose largest entry is below the smallest normal float is
+      // stored as zero: 1/mx would overflow and 0 * inf give NaN
+      if (!(mx >= FLT_MIN)) mx = R(0);
+      const R inv = mx > R(0) ? R(1) / mx : R(0);
+      norm[(size_t)j * nstride + s] = mx;
+#pragma unroll
+      for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(v[k]));
+    }
+  }
+};
+
 // grid of the reduction kernels: depends only on the volume
 int reduce_grid(long V);
 // sum n_sums partial arrays of length nparts into out (device), one workgroup

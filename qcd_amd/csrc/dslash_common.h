@@ -119,6 +119,9 @@ struct SpinorOut {
       R mx = R(0);
 #pragma unroll
       for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+      // a site whose largest entry is below the smallest normal float is
+      // stored as zero: 1/mx would overflow and 0 * inf give NaN
+      if (!(mx >= FLT_MIN)) mx = R(0);
       const R inv = mx > R(0) ? R(1) / mx : R(0);
       st_plane(norm, s * 4u, (float)mx);
 #pragma unroll

@@ -16,6 +16,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>

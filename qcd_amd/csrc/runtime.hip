@@ -154,6 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_half_spinor(const double2 *__
     mx = fmax(mx, fmax(fabs(v[k].x), fabs(v[k].y)));
   }
   float n = (float)mx;
+  if (!(n >= FLT_MIN)) n = 0.f;   // 1/n must stay finite
   float inv = n > 0.f ? 1.0f / n : 0.f;
   norm[s] = n;
 #pragma unroll
@@ -202,8 +203,10 @@ __global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict
   }
   double inv = 1.0;
   if (dnorm) {
-    dnorm[s] = (float)mx;
-    inv = mx > 0.0 ? 1.0 / (double)(float)mx : 0.0;
+    float n = (float)mx;
+    if (!(n >= FLT_MIN)) n = 0.f;   // 1/n must stay finite
+    dnorm[s] = n;
+    inv = n > 0.f ? 1.0 / (double)n : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < NC; k++) dst[elem(NC, k, s, stride)] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
