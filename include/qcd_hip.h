@@ -122,6 +122,8 @@ typedef struct {
    int    op;          /* QCD_HIP_OP_Q (as invert.c) or QCD_HIP_OP_D */
    int    keep_q1;     /* 1: drop the first GCR direction from the cycle's
                           solution exactly as invert.c:50-53 does */
+   int    batch;       /* right-hand sides solved together (1 = one at a
+                          time as invert.c); workspace grows with it */
 } qcd_hip_solver_params;
 typedef struct {
    int    outer;        /* outer cycles (summed over rhs) */

@@ -23,11 +23,14 @@ namespace qcdhip {
 
 template <int PS, int PG, int EPI, bool RED, int OCC>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
-k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, SpinorIn<PS> b,
+k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
          double *__restrict__ partials) {
   using R = typename Prec<PS>::R;
   const Site st = site_of_lane(a);
   const unsigned s = st.s;
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const SpinorIn<PS> &in = rv.in, &b = rv.b;
+  const SpinorOut<PS> &out = rv.out;
   double red = 0.0;
   if (st.valid) {
     // same summation order as k_dslash_zm, so every traversal gives the same
@@ -74,7 +77,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, Spinor
     }
     out.store(s, o);
   }
-  if (RED) block_reduce_store(red, partials, blockIdx.x);
+  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // z-marching Dslash: lane = one (t, x, y) column, steps z0 .. z0+zlen-1.
@@ -86,9 +89,12 @@ k_dslash(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, Spinor
 // acc holds -2 diag(psi) + sum of hops, so out = -1/2 acc.
 template <int PS, int PG, int EPI, bool RED, int OCC>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
-k_dslash_zm(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, SpinorIn<PS> b,
+k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
             double *__restrict__ partials) {
   using R = typename Prec<PS>::R;
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const SpinorIn<PS> &in = rv.in, &b = rv.b;
+  const SpinorOut<PS> &out = rv.out;
   const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const unsigned cb = lb % a.col_blocks, zc = lb / a.col_blocks;
   const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
@@ -161,19 +167,20 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in, GaugeIn<PG> U, SpinorOut<PS> out, Spi
       out.store(s, o);
     }
   }
-  if (RED) block_reduce_store(red, partials, blockIdx.x);
+  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 template <int PS, int PG, int EPI, bool RED, int OCC>
 static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
+  const dim3 gr((unsigned)grid, (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
-    k_dslash_zm<PS, PG, EPI, RED, OCC><<<grid, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                                    SpinorOut<PS>::of(out, ro), bi, partials);
+    k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
+                                                                  SpinorOut<PS>::of(out, ro), bi, partials);
   else
-    k_dslash<PS, PG, EPI, RED, OCC><<<grid, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                                 SpinorOut<PS>::of(out, ro), bi, partials);
+    k_dslash<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
+                                                               SpinorOut<PS>::of(out, ro), bi, partials);
   QH_CHECK(hipGetLastError());
   return 0;
 }
@@ -223,6 +230,9 @@ DslashArgs make_args(qcd_hip_lattice lat, double m, double tm) {
   a.mp4 = m + 4.0;
   a.tm = tm;
   a.map = MAP_XCD;
+  a.ids = nullptr;
+  a.in_rs = a.out_rs = a.b_rs = 0;
+  a.count = 1;
   return a;
 }
 
@@ -284,11 +294,22 @@ static TuneKey key_of(qcd_hip_field in, qcd_hip_field g) {
   return k;
 }
 
+struct Batch {
+  const int *ids;
+  int count, in_rs, out_rs, b_rs;
+};
+static const Batch kOne{nullptr, 1, 0, 0, 0};
+
 static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double m, double tm, double *partials,
-                      hipStream_t stream, int *grid_out) {
+                      hipStream_t stream, int *grid_out, const Batch &bt = kOne) {
   DslashArgs a = make_args(in->lat, m, tm);
   const int grid = set_traversal(a, map);
+  a.ids = bt.ids;
+  a.count = (unsigned)bt.count;
+  a.in_rs = (unsigned)bt.in_rs;
+  a.out_rs = (unsigned)bt.out_rs;
+  a.b_rs = (unsigned)bt.b_rs;
   if (grid_out) *grid_out = grid;
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
@@ -398,10 +419,30 @@ int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out);
 }
 
+int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
+                        qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m,
+                        double tm, double *partials, hipStream_t stream, int *grid_out) {
+  if (count < 1) return 0;
+  const TuneVal *t = tune_lookup(key_of(in, g));
+  const int map = t ? t->map : kDefaultMap, occ = t ? t->occ : 0;
+  const Batch bt{ids, count, in_rs, out_rs, b_rs};
+  return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out, bt);
+}
+
 // Tune (once per key) before the first use, outside any graph capture.
 int dslash_prepare(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
   if (!autotune_enabled()) return 0;
   return dslash_tune(out, in, g, 0);
+}
+
+// an upper bound of the Dslash grid over every traversal candidate
+int dslash_grid_bound(qcd_hip_lattice lat) {
+  int most = 0;
+  for (int map : {(int)MAP_XCD, (int)MAP_SLAB, MAP_ZMARCH | (1 << 8)}) {   // z-march of 1 has the most groups
+    DslashArgs a = make_args(lat, 0.0, 0.0);
+    most = std::max(most, set_traversal(a, map));
+  }
+  return most;
 }
 
 int dslash_grid(qcd_hip_field in, qcd_hip_field g) {
@@ -425,8 +466,8 @@ int qcd_hip_dslash(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, dou
   }
   const int epi = op == QCD_HIP_OP_Q ? EPI_Q : EPI_D;
   QH_TRY(dslash_prepare(out, in, gauge));
-  for (int r = 0; r < in->nrhs; r++)
-    QH_TRY(dslash_launch(epi, false, out, r, in, r, gauge, nullptr, 0, m, tm, nullptr, ctx()->stream, nullptr));
+  QH_TRY(dslash_launch_batch(epi, false, out, 0, 1, in, 0, 1, gauge, nullptr, 0, 0, nullptr, in->nrhs, m, tm, nullptr,
+                             ctx()->stream, nullptr));
   QH_CHECK(hipStreamSynchronize(ctx()->stream));
   return 0;
 }
@@ -443,13 +484,15 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
   const int epi = op == QCD_HIP_OP_Q ? EPI_Q : EPI_D;
   QH_TRY(dslash_prepare(out, in, gauge));
   for (int i = 0; i < warmup; i++)
-    for (int r = 0; r < in->nrhs; r++) QH_TRY(dslash_launch(epi, false, out, r, in, r, gauge, nullptr, 0, m, tm, nullptr, st, nullptr));
+    QH_TRY(dslash_launch_batch(epi, false, out, 0, 1, in, 0, 1, gauge, nullptr, 0, 0, nullptr, in->nrhs, m, tm, nullptr,
+                               st, nullptr));
   hipEvent_t e0, e1;
   QH_CHECK(hipEventCreate(&e0));
   QH_CHECK(hipEventCreate(&e1));
   QH_CHECK(hipEventRecord(e0, st));
   for (int i = 0; i < iters; i++)
-    for (int r = 0; r < in->nrhs; r++) QH_TRY(dslash_launch(epi, false, out, r, in, r, gauge, nullptr, 0, m, tm, nullptr, st, nullptr));
+    QH_TRY(dslash_launch_batch(epi, false, out, 0, 1, in, 0, 1, gauge, nullptr, 0, 0, nullptr, in->nrhs, m, tm, nullptr,
+                               st, nullptr));
   QH_CHECK(hipEventRecord(e1, st));
   QH_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -72,12 +72,23 @@ struct SpinorIn {
   using R = typename Prec<P>::R;
   const T *p[12];        // component k of tile 0
   const float *norm;
+  unsigned vstride;      // elements per vector (advance by whole vectors)
   static SpinorIn of(qcd_hip_field f, int rhs) {
     SpinorIn s;
     const size_t st = (size_t)f->lat->stride;
     const T *b = (const T *)f->data + (size_t)rhs * 12 * st;
     for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
     s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
+    s.vstride = (unsigned)(12 * st);
+    return s;
+  }
+  // the same accessor moved forward by nv vectors of the field
+  __device__ __forceinline__ SpinorIn advanced(unsigned nv) const {
+    SpinorIn s = *this;
+    const size_t e = (size_t)nv * vstride;
+#pragma unroll
+    for (int k = 0; k < 12; k++) s.p[k] += e;
+    if (norm) s.norm += e / 12;
     return s;
   }
   // byte offset of site s from its tile-0 component pointer
@@ -105,12 +116,22 @@ struct SpinorOut {
   using R = typename Prec<P>::R;
   T *p[12];
   float *norm;
+  unsigned vstride;
   static SpinorOut of(qcd_hip_field f, int rhs) {
     SpinorOut s;
     const size_t st = (size_t)f->lat->stride;
     T *b = (T *)f->data + (size_t)rhs * 12 * st;
     for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
     s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
+    s.vstride = (unsigned)(12 * st);
+    return s;
+  }
+  __device__ __forceinline__ SpinorOut advanced(unsigned nv) const {
+    SpinorOut s = *this;
+    const size_t e = (size_t)nv * vstride;
+#pragma unroll
+    for (int k = 0; k < 12; k++) s.p[k] += e;
+    if (norm) s.norm += e / 12;
     return s;
   }
   __device__ __forceinline__ void store(unsigned s, const Cx<R> v[12]) const {
@@ -177,7 +198,29 @@ struct DslashArgs {
   unsigned slab_blocks;            // MAP_SLAB: workgroups per XCD
   unsigned zlen;                   // MAP_ZMARCH: z steps per lane
   unsigned col_blocks;             // MAP_ZMARCH: workgroups per z chunk
+  // right-hand-side batch: grid row y works on slot = ids ? ids[y] : y, i.e.
+  // on vectors in + slot*in_rs, out + slot*out_rs, b + slot*b_rs
+  const int *ids;
+  unsigned in_rs, out_rs, b_rs;
+  unsigned count;                  // grid rows (host side)
 };
+
+template <int PS>
+struct RhsView {
+  SpinorIn<PS> in, b;
+  SpinorOut<PS> out;
+};
+
+template <int PS>
+__device__ __forceinline__ RhsView<PS> rhs_view(const DslashArgs &a, const SpinorIn<PS> &in, const SpinorOut<PS> &out,
+                                                const SpinorIn<PS> &b) {
+  const unsigned slot = a.ids ? (unsigned)a.ids[blockIdx.y] : blockIdx.y;
+  RhsView<PS> v;
+  v.in = in.advanced(slot * a.in_rs);
+  v.out = out.advanced(slot * a.out_rs);
+  v.b = b.advanced(slot * a.b_rs);
+  return v;
+}
 
 struct Site {
   unsigned s, t, x, y, z;
@@ -422,6 +465,13 @@ __device__ __forceinline__ void block_reduce_store(double v, double *partials, u
 }
 
 // launchers defined in dslash.hip
+// Batched launch over `count` right-hand sides (see DslashArgs::ids): row y
+// applies the operator to vector ri + slot*in_rs of `in`, writes vector
+// ro + slot*out_rs of `out`, reads vector rb + slot*b_rs of `b` (residual
+// epilogues); with red, |out|^2 partials go to partials[y * grid + x].
+int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
+                        qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m,
+                        double tm, double *partials, hipStream_t stream, int *grid_out);
 DslashArgs make_args(qcd_hip_lattice lat, double m, double tm);
 int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
 // asynchronous single-rhs launch; *grid_out (if given) receives the number of
@@ -432,5 +482,7 @@ int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in
 int dslash_prepare(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
 // number of workgroups dslash_launch will use for (in, g)
 int dslash_grid(qcd_hip_field in, qcd_hip_field g);
+// upper bound of that number over every launch configuration
+int dslash_grid_bound(qcd_hip_lattice lat);
 
 }  // namespace qcdhip

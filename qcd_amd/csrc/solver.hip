@@ -17,6 +17,9 @@
 // After n iterations one multi-axpy forms x = sum_j beta_j res_j with
 // beta_j = sum_{k>=j} c_k a[k][j] (the reference updates x every iteration).
 // All coefficients live in device memory: a cycle needs no host round trip.
+// Right-hand sides are solved in batches of prm.batch that advance in
+// lockstep: every kernel takes a second grid dimension over the batch, and
+// right-hand sides that have converged drop out of the launches.
 // Orthogonalisation is classical Gram-Schmidt in one fused pass (the
 // reference's modified Gram-Schmidt, invert.c:68-73, is the same projection in
 // exact arithmetic); the outer FP64 residual absorbs the difference.
@@ -68,49 +71,58 @@ __device__ __forceinline__ void group_sum(double (&v)[NV], double *out, unsigned
   __syncthreads();
 }
 
-// d_j = <w_j, w_k> for j < k; partials [2k][nb]
+// Batched kernels: grid row y works on right-hand-side slot ids[y]; the
+// Krylov vector j of slot b is vector b*n + j of the Rs / Ws fields; row y
+// writes its partial sums at part + y*rowsz, laid out [value][workgroup].
+
+// d_j = <w_j, w_k> for j < k
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_multidot(SpinorSet<P> W, int k, unsigned V, double *part) {
+__global__ void __launch_bounds__(kBlock) k_multidot(SpinorSet<P> W, int n, int k, const int *ids, unsigned V,
+                                                     double *part, size_t rowsz) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
+  double *pr = part + blockIdx.y * rowsz;
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   const bool ok = s < V;
   Cx<R> wk[12];
-  if (ok) W.load(k, s, wk);
+  if (ok) W.load(b * n + k, s, wk);
   for (int j = 0; j < k; j++) {
     double v[2] = {0.0, 0.0};
     if (ok) {
       Cx<R> wj[12];
-      W.load(j, s, wj);
+      W.load(b * n + j, s, wj);
 #pragma unroll
       for (int c = 0; c < 12; c++) {
         v[0] += (double)wj[c].re * (double)wk[c].re + (double)wj[c].im * (double)wk[c].im;
         v[1] += (double)wj[c].re * (double)wk[c].im - (double)wj[c].im * (double)wk[c].re;
       }
     }
-    group_sum<2>(v, part + (size_t)2 * j * gridDim.x, gridDim.x, blockIdx.x);
+    group_sum<2>(v, pr + (size_t)2 * j * gridDim.x, gridDim.x, blockIdx.x);
   }
 }
 
-// w_k += sum_j coef_j w_j; partials [3][nb] = |w_k|^2, Re/Im <w_k, src>
+// w_k += sum_j coef_j w_j; partials: |w_k|^2, Re/Im <w_k, src = res_0>
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, int k, const GcrScal *sc, SpinorSet<P> S,
-                                                           int js, unsigned V, double *part) {
+__global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, SpinorSet<P> Rs, int n, int k,
+                                                           const GcrScal *sc, const int *ids, unsigned V, double *part,
+                                                           size_t rowsz) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
   __shared__ double2 cf[kN];
-  for (int j = threadIdx.x; j < k; j += kBlock) cf[j] = sc->coef[j];
+  for (int j = threadIdx.x; j < k; j += kBlock) cf[j] = sc[b].coef[j];
   __syncthreads();
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (s < V) {
     Cx<R> wk[12];
-    W.load(k, s, wk);
+    W.load(b * n + k, s, wk);
     if (k > 0) {
       Cx<double> acc[12];
 #pragma unroll
       for (int c = 0; c < 12; c++) acc[c] = Cx<double>{(double)wk[c].re, (double)wk[c].im};
       for (int j = 0; j < k; j++) {
         Cx<R> wj[12];
-        W.load(j, s, wj);
+        W.load(b * n + j, s, wj);
         const double2 a = cf[j];
 #pragma unroll
         for (int c = 0; c < 12; c++) {
@@ -120,11 +132,11 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, int k
       }
 #pragma unroll
       for (int c = 0; c < 12; c++) wk[c] = Cx<R>{(R)acc[c].re, (R)acc[c].im};
-      W.store(k, s, wk);
-      if constexpr (P == QCD_HIP_F16) W.load(k, s, wk);   // norms of the stored (scaled) vector
+      W.store(b * n + k, s, wk);
+      if constexpr (P == QCD_HIP_F16) W.load(b * n + k, s, wk);   // norms of the stored (scaled) vector
     }
     Cx<R> sv[12];
-    S.load(js, s, sv);
+    Rs.load(b * n, s, sv);
 #pragma unroll
     for (int c = 0; c < 12; c++) {
       const double wr = wk[c].re, wi = wk[c].im;
@@ -133,35 +145,37 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, int k
       v[2] += wr * (double)sv[c].im - wi * (double)sv[c].re;
     }
   }
-  group_sum<3>(v, part, gridDim.x, blockIdx.x);
+  group_sum<3>(v, part + blockIdx.y * rowsz, gridDim.x, blockIdx.x);
 }
 
 // res_{k+1} = res_k + alpha w_k
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_res_update(SpinorSet<P> Rs, SpinorSet<P> W, int k, const GcrScal *sc,
-                                                       unsigned V) {
+__global__ void __launch_bounds__(kBlock) k_res_update(SpinorSet<P> Rs, SpinorSet<P> W, int n, int k,
+                                                       const GcrScal *sc, const int *ids, unsigned V) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
-  const double2 al = sc->alpha;
+  const double2 al = sc[b].alpha;
   Cx<R> r[12], w[12];
-  Rs.load(k, s, r);
-  W.load(k, s, w);
+  Rs.load(b * n + k, s, r);
+  W.load(b * n + k, s, w);
 #pragma unroll
   for (int c = 0; c < 12; c++) {
     const double wr = w[c].re, wi = w[c].im;
     r[c] = Cx<R>{(R)((double)r[c].re + al.x * wr - al.y * wi), (R)((double)r[c].im + al.x * wi + al.y * wr)};
   }
-  Rs.store(k + 1, s, r);
+  Rs.store(b * n + k + 1, s, r);
 }
 
-// e = sum_j beta_j res_j
+// e_b = sum_j beta_j res_j
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, const GcrScal *sc, SpinorSet<P> E, int je,
-                                                     unsigned V) {
+__global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, const GcrScal *sc, SpinorSet<P> E,
+                                                     const int *ids, unsigned V) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
   __shared__ double2 bt[kN];
-  for (int j = threadIdx.x; j < n; j += kBlock) bt[j] = sc->beta[j];
+  for (int j = threadIdx.x; j < n; j += kBlock) bt[j] = sc[b].beta[j];
   __syncthreads();
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
@@ -170,18 +184,18 @@ __global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, con
   for (int c = 0; c < 12; c++) acc[c] = Cx<double>{0.0, 0.0};
   for (int j = 0; j < n; j++) {
     Cx<R> r[12];
-    Rs.load(j, s, r);
-    const double2 b = bt[j];
+    Rs.load(b * n + j, s, r);
+    const double2 bb = bt[j];
 #pragma unroll
     for (int c = 0; c < 12; c++) {
-      acc[c].re += b.x * (double)r[c].re - b.y * (double)r[c].im;
-      acc[c].im += b.x * (double)r[c].im + b.y * (double)r[c].re;
+      acc[c].re += bb.x * (double)r[c].re - bb.y * (double)r[c].im;
+      acc[c].im += bb.x * (double)r[c].im + bb.y * (double)r[c].re;
     }
   }
   Cx<R> o[12];
 #pragma unroll
   for (int c = 0; c < 12; c++) o[c] = Cx<R>{(R)acc[c].re, (R)acc[c].im};
-  E.store(je, s, o);
+  E.store(b, s, o);
 }
 
 // Sum a [nv][nb] partial array (one wave per value, fixed order) into sums.
@@ -196,10 +210,12 @@ __device__ void wave_sums(const double *part, int nv, unsigned nb, double *sums)
   }
 }
 
-// after the multi-dot of iteration k
-__global__ void __launch_bounds__(1024) k_scal_dots(GcrScal *sc, const double *part, unsigned nb, int k, int n) {
+// after the multi-dot of iteration k (one workgroup per slot)
+__global__ void __launch_bounds__(1024) k_scal_dots(GcrScal *scs, const double *part, size_t rowsz, unsigned nb, int k,
+                                                    int n, const int *ids) {
   __shared__ double sums[2 * kN];
-  wave_sums(part, 2 * k, nb, sums);
+  GcrScal *sc = scs + ids[blockIdx.x];
+  wave_sums(part + blockIdx.x * rowsz, 2 * k, nb, sums);
   __syncthreads();
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
     const double sj = sc->s[j];
@@ -211,9 +227,11 @@ __global__ void __launch_bounds__(1024) k_scal_dots(GcrScal *sc, const double *p
 
 // after the multi-axpy of iteration k: normalisation, c_k and the
 // coefficient recurrence of invert.c:83-94
-__global__ void __launch_bounds__(1024) k_scal_norm(GcrScal *sc, const double *part, unsigned nb, int k, int n) {
+__global__ void __launch_bounds__(1024) k_scal_norm(GcrScal *scs, const double *part, size_t rowsz, unsigned nb, int k,
+                                                    int n, const int *ids) {
   __shared__ double sums[3];
-  wave_sums(part, 3, nb, sums);
+  GcrScal *sc = scs + ids[blockIdx.x];
+  wave_sums(part + blockIdx.x * rowsz, 3, nb, sums);
   __syncthreads();
   if (threadIdx.x != 0) return;
   const double sk = 1.0 / sqrt(sums[0]);
@@ -241,7 +259,8 @@ __global__ void __launch_bounds__(1024) k_scal_norm(GcrScal *sc, const double *p
 
 // beta_j = sum_{k >= j} c_k a[k][j]; keep_q1 drops the k = 0 term as
 // invert.c:50-53 does
-__global__ void k_scal_beta(GcrScal *sc, int n, int keep_q1) {
+__global__ void k_scal_beta(GcrScal *scs, int n, int keep_q1, const int *ids) {
+  GcrScal *sc = scs + ids[blockIdx.x];
   const int j = threadIdx.x;
   if (j >= n) return;
   double2 b = make_double2(0.0, 0.0);
@@ -253,61 +272,74 @@ __global__ void k_scal_beta(GcrScal *sc, int n, int keep_q1) {
   sc->beta[j] = b;
 }
 
-// outer residual norm from Dslash partials
-__global__ void __launch_bounds__(1024) k_scal_rnorm(GcrScal *sc, const double *part, unsigned nb) {
+// outer residual norms from the Dslash partials (row x of the launch)
+__global__ void __launch_bounds__(1024) k_scal_rnorm(GcrScal *scs, const double *part, unsigned nb, const int *ids,
+                                                     double *rn) {
   __shared__ double sums[1];
-  wave_sums(part, 1, nb, sums);
+  const int b = ids[blockIdx.x];
+  wave_sums(part + (size_t)blockIdx.x * nb, 1, nb, sums);
   __syncthreads();
   if (threadIdx.x == 0) {
+    GcrScal *sc = scs + b;
     sc->rnorm = sqrt(sums[0]);
     sc->rscale = sc->rnorm > 0.0 ? 1.0 / sc->rnorm : 0.0;
+    rn[b] = sc->rnorm;
   }
 }
 
 // res_0 = r / |r| in the inner precision
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_prepare_src(SpinorIn<QCD_HIP_F64> r, const GcrScal *sc, SpinorSet<P> Rs,
-                                                        unsigned V) {
+__global__ void __launch_bounds__(kBlock) k_prepare_src(SpinorSet<QCD_HIP_F64> r, const GcrScal *sc, SpinorSet<P> Rs,
+                                                        int n, const int *ids, unsigned V) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
-  const double f = sc->rscale;
+  const double f = sc[b].rscale;
   Cx<double> v[12];
-  r.load(s, v);
+  r.load(b, s, v);
   Cx<R> o[12];
 #pragma unroll
   for (int c = 0; c < 12; c++) o[c] = Cx<R>{(R)(v[c].re * f), (R)(v[c].im * f)};
-  Rs.store(0, s, o);
+  Rs.store(b * n, s, o);
 }
 
-// x += |r| e (FP64 accumulation of the inner correction)
+// x_{x0+b} += |r_b| e_b (FP64 accumulation of the inner correction)
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_accumulate(SpinorIn<QCD_HIP_F64> xin, SpinorOut<QCD_HIP_F64> xout,
-                                                       SpinorSet<P> E, int je, const GcrScal *sc, unsigned V) {
+__global__ void __launch_bounds__(kBlock) k_accumulate(SpinorSet<QCD_HIP_F64> X, int x0, SpinorSet<P> E,
+                                                       const GcrScal *sc, const int *ids, unsigned V) {
   using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
-  const double f = sc->rnorm;
+  const double f = sc[b].rnorm;
   Cx<double> x[12];
   Cx<R> e[12];
-  xin.load(s, x);
-  E.load(je, s, e);
+  X.load(x0 + b, s, x);
+  E.load(b, s, e);
 #pragma unroll
   for (int c = 0; c < 12; c++) x[c] = Cx<double>{x[c].re + f * (double)e[c].re, x[c].im + f * (double)e[c].im};
-  xout.store(s, x);
+  X.store(x0 + b, s, x);
 }
 
-// out = i g5 in (spins 0,1 times i, spins 2,3 times -i)
-__global__ void __launch_bounds__(kBlock) k_ig5(SpinorIn<QCD_HIP_F64> in, SpinorOut<QCD_HIP_F64> out, unsigned V) {
+// system right-hand side of slot b (i g5 b for Q, b for D) and x = 0
+__global__ void __launch_bounds__(kBlock) k_start(SpinorSet<QCD_HIP_F64> B, SpinorSet<QCD_HIP_F64> X, int r0,
+                                                  SpinorSet<QCD_HIP_F64> Bq, int use_q, unsigned V) {
+  const int b = blockIdx.y;
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
   Cx<double> v[12];
-  in.load(s, v);
+  B.load(r0 + b, s, v);
+  if (use_q) {
 #pragma unroll
-  for (int c = 0; c < 6; c++) v[c] = ipow<1>(v[c]);
+    for (int c = 0; c < 6; c++) v[c] = ipow<1>(v[c]);
 #pragma unroll
-  for (int c = 6; c < 12; c++) v[c] = ipow<3>(v[c]);
-  out.store(s, v);
+    for (int c = 6; c < 12; c++) v[c] = ipow<3>(v[c]);
+  }
+  Bq.store(b, s, v);
+#pragma unroll
+  for (int c = 0; c < 12; c++) v[c] = Cx<double>{0.0, 0.0};
+  X.store(r0 + b, s, v);
 }
 
 }  // namespace qcdhip
@@ -318,77 +350,140 @@ using namespace qcdhip;
 struct qcd_hip_solver_s {
   qcd_hip_lattice lat;
   qcd_hip_solver_params prm;
+  int B = 1;                        // right-hand sides per batch
   qcd_hip_field g64 = nullptr;      // caller's FP64 gauge field (not owned)
   qcd_hip_field gin = nullptr;      // inner-precision copy (owned unless F64)
-  qcd_hip_field Rs = nullptr;       // n search directions res_j (inner precision)
-  qcd_hip_field Ws = nullptr;       // n vectors w_j = A res_j (inner precision)
-  qcd_hip_field E = nullptr;        // inner correction e (inner precision)
-  qcd_hip_field r64 = nullptr;      // FP64 residual
-  qcd_hip_field bq = nullptr;       // FP64 right-hand side of the solved system
-  GcrScal *sc = nullptr;
+  qcd_hip_field Rs = nullptr;       // n*B search directions res_j (inner precision)
+  qcd_hip_field Ws = nullptr;       // n*B vectors w_j = A res_j (inner precision)
+  qcd_hip_field E = nullptr;        // B inner corrections (inner precision)
+  qcd_hip_field r64 = nullptr;      // B FP64 residuals
+  qcd_hip_field bq = nullptr;       // B FP64 right-hand sides of the solved system
+  GcrScal *sc = nullptr;            // B coefficient sets
+  double *rn = nullptr;             // B residual norms (device)
+  double *rn_host = nullptr;        // pinned mirror
+  int *ids = nullptr;               // active slots (device)
+  int *ids_host = nullptr;          // pinned mirror
   double *part = nullptr;
-  size_t part_bytes = 0;
+  size_t part_bytes = 0, rowsz = 0;
   unsigned nb = 0;                   // site-kernel workgroups
 };
 
 namespace {
 
 template <int P>
-int gcr_cycle(qcd_hip_solver_s *S, double kappa, double mu, hipStream_t st) {
+int gcr_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream_t st) {
   const int n = S->prm.n_restart;
   const unsigned V = (unsigned)S->lat->V, nb = S->nb;
   const int epi = S->prm.op == QCD_HIP_OP_Q ? EPI_Q : EPI_D;
   const double m = 1.0 / (2.0 * kappa) - 4.0;
   SpinorSet<P> Rs = SpinorSet<P>::of(S->Rs), Ws = SpinorSet<P>::of(S->Ws), E = SpinorSet<P>::of(S->E);
+  const dim3 g2(nb, (unsigned)count);
   for (int k = 0; k < n; k++) {
-    QH_TRY(dslash_launch(epi, false, S->Ws, k, S->Rs, k, S->gin, nullptr, 0, m, mu, nullptr, st, nullptr));
+    QH_TRY(dslash_launch_batch(epi, false, S->Ws, k, n, S->Rs, k, n, S->gin, nullptr, 0, 0, S->ids, count, m, mu,
+                               nullptr, st, nullptr));
     if (k > 0) {
-      k_multidot<P><<<nb, kBlock, 0, st>>>(Ws, k, V, S->part);
-      k_scal_dots<<<1, 1024, 0, st>>>(S->sc, S->part, nb, k, n);
+      k_multidot<P><<<g2, kBlock, 0, st>>>(Ws, n, k, S->ids, V, S->part, S->rowsz);
+      k_scal_dots<<<count, 1024, 0, st>>>(S->sc, S->part, S->rowsz, nb, k, n, S->ids);
     }
-    k_multiaxpy_norm<P><<<nb, kBlock, 0, st>>>(Ws, k, S->sc, Rs, 0, V, S->part);
-    k_scal_norm<<<1, 1024, 0, st>>>(S->sc, S->part, nb, k, n);
-    if (k + 1 < n) k_res_update<P><<<nb, kBlock, 0, st>>>(Rs, Ws, k, S->sc, V);
+    k_multiaxpy_norm<P><<<g2, kBlock, 0, st>>>(Ws, Rs, n, k, S->sc, S->ids, V, S->part, S->rowsz);
+    k_scal_norm<<<count, 1024, 0, st>>>(S->sc, S->part, S->rowsz, nb, k, n, S->ids);
+    if (k + 1 < n) k_res_update<P><<<g2, kBlock, 0, st>>>(Rs, Ws, n, k, S->sc, S->ids, V);
   }
-  k_scal_beta<<<1, kN, 0, st>>>(S->sc, n, S->prm.keep_q1);
-  k_solution<P><<<nb, kBlock, 0, st>>>(Rs, n, S->sc, E, 0, V);
+  k_scal_beta<<<count, kN, 0, st>>>(S->sc, n, S->prm.keep_q1, S->ids);
+  k_solution<P><<<g2, kBlock, 0, st>>>(Rs, n, S->sc, E, S->ids, V);
   QH_CHECK(hipGetLastError());
   return 0;
 }
 
-int inner_cycle(qcd_hip_solver_s *S, double kappa, double mu, hipStream_t st) {
-  switch (S->prm.inner_prec) {
-    case QCD_HIP_F64: return gcr_cycle<QCD_HIP_F64>(S, kappa, mu, st);
-    case QCD_HIP_F32: return gcr_cycle<QCD_HIP_F32>(S, kappa, mu, st);
-    default: return gcr_cycle<QCD_HIP_F16>(S, kappa, mu, st);
-  }
-}
-
 template <int P>
-void launch_prepare(qcd_hip_solver_s *S, hipStream_t st) {
-  k_prepare_src<P><<<S->nb, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F64>::of(S->r64, 0), S->sc, SpinorSet<P>::of(S->Rs),
-                                             (unsigned)S->lat->V);
+int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
+  const dim3 g2(S->nb, (unsigned)count);
+  const unsigned V = (unsigned)S->lat->V;
+  k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc, SpinorSet<P>::of(S->Rs),
+                                          S->prm.n_restart, S->ids, V);
+  QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
+  k_accumulate<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(x), x0, SpinorSet<P>::of(S->E), S->sc, S->ids, V);
+  QH_CHECK(hipGetLastError());
+  return 0;
 }
 
-template <int P>
-void launch_accumulate(qcd_hip_solver_s *S, qcd_hip_field x, int rx, hipStream_t st) {
-  k_accumulate<P><<<S->nb, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F64>::of(x, rx), SpinorOut<QCD_HIP_F64>::of(x, rx),
-                                            SpinorSet<P>::of(S->E), 0, S->sc, (unsigned)S->lat->V);
-}
-
-// r = bq - A x with |r|^2 into sc->rnorm; returns |r| on the host
-int true_residual(qcd_hip_solver_s *S, qcd_hip_field x, int rx, double kappa, double mu, hipStream_t st,
-                  double *rnorm) {
+// r_b = bq_b - A x_{x0+b} for the active slots; residual norms to rn_host
+int true_residual(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
   const int epi = S->prm.op == QCD_HIP_OP_Q ? EPI_RES_Q : EPI_RES_D;
   int grid = 0;
-  QH_TRY(dslash_launch(epi, true, S->r64, 0, x, rx, S->g64, S->bq, 0, 1.0 / (2.0 * kappa) - 4.0, mu, S->part, st,
-                       &grid));
-  k_scal_rnorm<<<1, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid);
+  QH_TRY(dslash_launch_batch(epi, true, S->r64, 0, 1, x, x0, 1, S->g64, S->bq, 0, 1, S->ids, count,
+                             1.0 / (2.0 * kappa) - 4.0, mu, S->part, st, &grid));
+  k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);
   QH_CHECK(hipGetLastError());
-  Context *c = ctx();
-  QH_CHECK(hipMemcpyAsync(c->host_scalars, &S->sc->rnorm, sizeof(double), hipMemcpyDeviceToHost, st));
+  QH_CHECK(hipMemcpyAsync(S->rn_host, S->rn, sizeof(double) * S->B, hipMemcpyDeviceToHost, st));
   QH_CHECK(hipStreamSynchronize(st));
-  *rnorm = c->host_scalars[0];
+  return 0;
+}
+
+int upload_ids(qcd_hip_solver_s *S, const std::vector<int> &ids, hipStream_t st) {
+  for (size_t i = 0; i < ids.size(); i++) S->ids_host[i] = ids[i];
+  QH_CHECK(hipMemcpyAsync(S->ids, S->ids_host, sizeof(int) * ids.size(), hipMemcpyHostToDevice, st));
+  QH_CHECK(hipStreamSynchronize(st));   // the pinned mirror is reused next step
+  return 0;
+}
+
+// Solve rhs [x0, x0 + cnt) of b into x.
+int solve_batch(qcd_hip_solver_s *S, qcd_hip_field x, qcd_hip_field b, int x0, int cnt, double kappa, double mu,
+                hipStream_t st, qcd_hip_solver_stats *tot, int *worst) {
+  const unsigned V = (unsigned)S->lat->V;
+  k_start<<<dim3(S->nb, (unsigned)cnt), kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(b), SpinorSet<QCD_HIP_F64>::of(x),
+                                                         x0, SpinorSet<QCD_HIP_F64>::of(S->bq),
+                                                         S->prm.op == QCD_HIP_OP_Q, V);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipMemsetAsync(S->sc, 0, sizeof(GcrScal) * S->B, st));
+  std::vector<int> act(cnt);
+  for (int i = 0; i < cnt; i++) act[i] = i;
+  QH_TRY(upload_ids(S, act, st));
+  QH_TRY(true_residual(S, x, x0, cnt, kappa, mu, st));
+  std::vector<double> bnorm(cnt), rel(cnt);
+  std::vector<int> outer(cnt, 0);
+  std::vector<GcrScal> host_sc;
+  for (int i = 0; i < cnt; i++) {
+    bnorm[i] = S->rn_host[i];
+    rel[i] = bnorm[i] > 0.0 ? 1.0 : 0.0;
+  }
+  int bad = 0;
+  for (int it = 0; it < S->prm.max_outer; it++) {
+    act.clear();
+    for (int i = 0; i < cnt; i++)
+      if (rel[i] > S->prm.tol) act.push_back(i);
+    if (act.empty()) break;
+    QH_TRY(upload_ids(S, act, st));
+    const int na = (int)act.size();
+    switch (S->prm.inner_prec) {
+      case QCD_HIP_F64: QH_TRY(outer_step<QCD_HIP_F64>(S, x, x0, na, kappa, mu, st)); break;
+      case QCD_HIP_F32: QH_TRY(outer_step<QCD_HIP_F32>(S, x, x0, na, kappa, mu, st)); break;
+      default: QH_TRY(outer_step<QCD_HIP_F16>(S, x, x0, na, kappa, mu, st)); break;
+    }
+    QH_TRY(true_residual(S, x, x0, na, kappa, mu, st));
+    for (int i : act) {
+      outer[i]++;
+      rel[i] = S->rn_host[i] / bnorm[i];
+      if (!(rel[i] == rel[i])) bad = 1;
+    }
+    int flag = 0;
+    for (int i : act) {
+      int f = 0;
+      QH_CHECK(hipMemcpy(&f, &S->sc[i].bad, sizeof(int), hipMemcpyDeviceToHost));
+      flag |= f;
+    }
+    if (flag || bad) {
+      bad = 1;
+      break;
+    }
+  }
+  for (int i = 0; i < cnt; i++) {
+    tot->outer += outer[i];
+    tot->inner_iters += outer[i] * S->prm.n_restart;
+    tot->relres = std::max(tot->relres, rel[i]);
+    if (!bad && rel[i] > S->prm.tol) *worst = std::max(*worst, 6);
+  }
+  if (bad) *worst = 5;
   return 0;
 }
 
@@ -404,31 +499,43 @@ void qcd_hip_solver_default_params(qcd_hip_solver_params *p) {
   p->inner_prec = QCD_HIP_F64;
   p->op = QCD_HIP_OP_Q;
   p->keep_q1 = 0;
+  p->batch = 1;
 }
 
 int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hip_solver_params *prm) {
   *out = nullptr;
   QH_TRY(ensure_init());
   if (!lat || !prm || prm->n_restart < 1 || prm->n_restart > kN || prm->tol <= 0.0 || prm->max_outer < 1 ||
-      prm->inner_prec < QCD_HIP_F64 || prm->inner_prec > QCD_HIP_F16 ||
+      prm->inner_prec < QCD_HIP_F64 || prm->inner_prec > QCD_HIP_F16 || prm->batch < 1 || prm->batch > 1024 ||
       (prm->op != QCD_HIP_OP_Q && prm->op != QCD_HIP_OP_D)) {
-    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1)", kN);
+    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1, batch 1..1024)", kN);
     return 1;
   }
   auto *S = new qcd_hip_solver_s();
   S->lat = lat;
   S->prm = *prm;
+  S->B = prm->batch;
   S->nb = (unsigned)grid_for(lat->V);
-  int rc = qcd_hip_spinor_create(&S->Rs, lat, prm->inner_prec, prm->n_restart);
-  if (!rc) rc = qcd_hip_spinor_create(&S->Ws, lat, prm->inner_prec, prm->n_restart);
-  if (!rc) rc = qcd_hip_spinor_create(&S->E, lat, prm->inner_prec, 1);
-  if (!rc) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, 1);
-  if (!rc) rc = qcd_hip_spinor_create(&S->bq, lat, QCD_HIP_F64, 1);
-  if (!rc) rc = dev_alloc((void **)&S->sc, sizeof(GcrScal));
-  // partials: 2 (n-1) per site workgroup (multi-dot) or one per Dslash workgroup
-  S->part_bytes = sizeof(double) * (size_t)std::max<size_t>(2 * (size_t)prm->n_restart + 4, 8) * (S->nb + 64);
+  const int n = prm->n_restart, B = S->B;
+  int rc = qcd_hip_spinor_create(&S->Rs, lat, prm->inner_prec, n * B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->Ws, lat, prm->inner_prec, n * B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->E, lat, prm->inner_prec, B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->bq, lat, QCD_HIP_F64, B);
+  if (!rc) rc = dev_alloc((void **)&S->sc, sizeof(GcrScal) * B);
+  if (!rc) rc = dev_alloc((void **)&S->rn, sizeof(double) * B);
+  if (!rc) rc = dev_alloc((void **)&S->ids, sizeof(int) * B);
+  if (!rc && hipHostMalloc((void **)&S->rn_host, sizeof(double) * B, hipHostMallocDefault) != hipSuccess) rc = 4;
+  if (!rc && hipHostMalloc((void **)&S->ids_host, sizeof(int) * B, hipHostMallocDefault) != hipSuccess) rc = 4;
+  // per batch row: 2(n-1) multi-dot sums per site workgroup, or one sum per
+  // Dslash workgroup (a z-march grid never exceeds site workgroups + T*Z)
+  const size_t rowmax = std::max<size_t>((size_t)2 * n + 4, 4) * ((size_t)S->nb + 64);
+  const size_t dsl = (size_t)dslash_grid_bound(lat);
+  S->rowsz = std::max(rowmax, dsl);
+  S->part_bytes = sizeof(double) * S->rowsz * B;
   if (!rc) rc = dev_alloc((void **)&S->part, S->part_bytes);
   if (rc) {
+    if (rc == 4) set_error("solver_create: out of memory for %d rhs x %d Krylov vectors", B, n);
     qcd_hip_solver_destroy(S);
     return rc;
   }
@@ -445,8 +552,12 @@ void qcd_hip_solver_destroy(qcd_hip_solver S) {
   qcd_hip_field_destroy(S->bq);
   if (S->gin && S->gin != S->g64) qcd_hip_field_destroy(S->gin);
   (void)hipStreamSynchronize(ctx()->stream);
-  dev_free(S->sc, sizeof(GcrScal));
+  dev_free(S->sc, sizeof(GcrScal) * S->B);
+  dev_free(S->rn, sizeof(double) * S->B);
+  dev_free(S->ids, sizeof(int) * S->B);
   dev_free(S->part, S->part_bytes);
+  if (S->rn_host) (void)hipHostFree(S->rn_host);
+  if (S->ids_host) (void)hipHostFree(S->ids_host);
   delete S;
 }
 
@@ -482,6 +593,10 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
     set_error("solver_solve: x and b must be FP64 spinors with equal rhs counts on the solver's lattice");
     return 1;
   }
+  if (x == b) {
+    set_error("solver_solve: x must not alias b");
+    return 1;
+  }
   if (kappa <= 0.0) {
     set_error("solver_solve: kappa must be positive");
     return 1;
@@ -491,46 +606,12 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
   // Tune the inner and outer operators before the first timed use.
   QH_TRY(dslash_prepare(S->Ws, S->Rs, S->gin));
   QH_TRY(dslash_prepare(S->r64, x, S->g64));
-  const unsigned V = (unsigned)S->lat->V;
-  int worst = 0;
   qcd_hip_solver_stats tot;
   std::memset(&tot, 0, sizeof(tot));
-  for (int r = 0; r < b->nrhs; r++) {
-    // system right-hand side: b, or i g5 b when solving with Q
-    if (S->prm.op == QCD_HIP_OP_Q)
-      k_ig5<<<S->nb, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F64>::of(b, r), SpinorOut<QCD_HIP_F64>::of(S->bq, 0), V);
-    else
-      QH_TRY(lincomb_launch(S->bq, 0, 1.0, 0.0, b, r, 0.0, 0.0, nullptr, 0, -1, st));
-    QH_TRY(lincomb_launch(x, r, 0.0, 0.0, x, r, 0.0, 0.0, nullptr, 0, -1, st));   // x = 0
-    QH_CHECK(hipMemsetAsync(S->sc, 0, sizeof(GcrScal), st));
-    double bnorm = 0.0, rn = 0.0;
-    QH_TRY(true_residual(S, x, r, kappa, mu, st, &bnorm));
-    int outer = 0, bad = 0;
-    double rel = bnorm > 0.0 ? 1.0 : 0.0;
-    while (rel > S->prm.tol && outer < S->prm.max_outer) {
-      if (S->prm.inner_prec == QCD_HIP_F64) launch_prepare<QCD_HIP_F64>(S, st);
-      else if (S->prm.inner_prec == QCD_HIP_F32) launch_prepare<QCD_HIP_F32>(S, st);
-      else launch_prepare<QCD_HIP_F16>(S, st);
-      QH_TRY(inner_cycle(S, kappa, mu, st));
-      if (S->prm.inner_prec == QCD_HIP_F64) launch_accumulate<QCD_HIP_F64>(S, x, r, st);
-      else if (S->prm.inner_prec == QCD_HIP_F32) launch_accumulate<QCD_HIP_F32>(S, x, r, st);
-      else launch_accumulate<QCD_HIP_F16>(S, x, r, st);
-      QH_CHECK(hipGetLastError());
-      QH_TRY(true_residual(S, x, r, kappa, mu, st, &rn));
-      outer++;
-      rel = rn / bnorm;
-      int flag = 0;
-      QH_CHECK(hipMemcpy(&flag, &S->sc->bad, sizeof(int), hipMemcpyDeviceToHost));
-      if (flag || !(rel == rel)) {
-        bad = 1;
-        break;
-      }
-    }
-    tot.outer += outer;
-    tot.inner_iters += outer * S->prm.n_restart;
-    tot.relres = std::max(tot.relres, rel);
-    if (bad) worst = std::max(worst, 5);
-    else if (rel > S->prm.tol) worst = std::max(worst, 6);
+  int worst = 0;
+  for (int r0 = 0; r0 < b->nrhs && worst != 5; r0 += S->B) {
+    const int cnt = std::min(S->B, b->nrhs - r0);
+    QH_TRY(solve_batch(S, x, b, r0, cnt, kappa, mu, st, &tot, &worst));
   }
   tot.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = tot;

@@ -54,13 +54,20 @@ def test_solve_multi_rhs_and_twisted(setup):
             s = Q.point_source(L, (1, 2, 3, 4), sp, c)
             srcs.append(s)
             fb.upload(s, rhs=3 * sp + c)
-    fx = Q.Spinor(lat, nrhs=12)
-    S = Q.Solver(lat, g, n_restart=24, tol=1e-10, inner_prec="f32")
-    st = S.solve(fx, fb, kappa, mu)
     geo = O.Geometry(L)
+    sols = {}
+    for batch in (1, 5, 12):   # one at a time (invert.c), ragged batches, all at once
+        fx = Q.Spinor(lat, nrhs=12)
+        S = Q.Solver(lat, g, n_restart=24, tol=1e-10, inner_prec="f32", batch=batch)
+        st = S.solve(fx, fb, kappa, mu)
+        assert st["outer"] >= 12
+        sols[batch] = [fx.download(r) for r in range(12)]
+        for r in range(12):
+            assert O.b_minus_Dx(geo, sols[batch][r], srcs[r], u, kappa, mu) <= 1e-10
+    # a right-hand side gets the same answer alone and inside a batch
     for r in range(12):
-        assert O.b_minus_Dx(geo, fx.download(r), srcs[r], u, kappa, mu) <= 1e-10
-    assert st["outer"] >= 12
+        assert np.array_equal(sols[1][r], sols[12][r])
+        assert np.array_equal(sols[5][r], sols[12][r])
 
 
 def test_reference_q1_variant(setup):
