@@ -37,6 +37,9 @@ int    qcd_hip_init(int device);
 int    qcd_hip_device_name(char *buf, int len);
 int    qcd_hip_synchronize(void);
 size_t qcd_hip_bytes_in_use(void);
+/* Device blocks freed by the library are cached for reuse (hipFree would
+ * synchronise the device); this hands the cache back to the runtime. */
+void   qcd_hip_release_cache(void);
 
 /* ------------------------------------------------------------ lattice
  * L = {T, X, Y, Z}; theta are the fermion boundary phases
@@ -64,6 +67,12 @@ int  qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src);
  * matrix from seeded N(0,1) draws; spinor entries i.i.d. N(0,1). */
 int qcd_hip_gauge_generate_host(double *aos, long V, double eps, uint64_t seed);
 int qcd_hip_spinor_gaussian_host(double *aos, long V, uint64_t seed);
+/* The same recipe run on the GPU straight into an FP64 gauge field (device
+ * math library: equal to the host field to rounding, not bit for bit). */
+int qcd_hip_gauge_generate(qcd_hip_field gauge64, double eps, uint64_t seed);
+/* rhs of an FP64 spinor field = delta at site {t,x,y,z} on (spin, col)
+ * (source.c:203-208) */
+int qcd_hip_point_source(qcd_hip_field f, int rhs, const int site[4], int spin, int col);
 
 /* ------------------------------------------------------------ operators
  * out = D in (op = QCD_HIP_OP_D) or Q in = i gamma5 D in (QCD_HIP_OP_Q),
@@ -147,6 +156,43 @@ int  qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field gauge64, dou
 /* ||D x[rx] - b[rb]|| / ||b[rb]|| in FP64, m = 1/(2 kappa) - 4 (b_minus_Dx.c:152-167) */
 int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
                         double kappa, double mu, double *rel);
+
+/* ------------------------------------------------------------ serving
+ * Continuous serving of propagator-solve requests (no reference
+ * counterpart; each request is the invert.c solve of its own configuration).
+ * Requests arrive at arrival_s seconds after the start (realtime != 0) or all
+ * at once (realtime == 0).  `workers` host threads, each with its own streams,
+ * take arrived requests in arrival order whenever a worker is free and the
+ * request's device footprint fits next to the running ones in
+ * `budget_bytes`; running requests overlap on the GPU.  A request generates
+ * its gauge field on the device (qcd_hip_gauge_generate recipe), puts nrhs
+ * point sources at the origin (rhs r: spin r/3, colour r%3) and solves them as
+ * one batch to tol, verified by the FP64 true residual.  solutions[i], when
+ * not NULL, receives request i's solutions as nrhs host AoS spinors. */
+typedef struct {
+   double   arrival_s;
+   int      L[4];
+   double   kappa;
+   double   eps;         /* gauge field U = reunitarize(exp(i eps H)) */
+   uint64_t seed;
+   int      nrhs;        /* 1 or 12 point sources */
+   int      inner_prec;  /* QCD_HIP_F64 plain, QCD_HIP_F32 / F16 mixed */
+   int      n_restart;
+   double   tol;
+} qcd_hip_request;
+typedef struct {
+   double queued_s;      /* arrival to start of service */
+   double service_s;     /* start of service to completion */
+   double latency_s;     /* arrival to completion */
+   int    rc;            /* 0 = every rhs converged */
+   int    outer;
+   int    iters;
+   double relres;        /* largest FP64 true relative residual over rhs */
+   double bytes;         /* device bytes reserved for the request */
+   double xnorm2;        /* sum over rhs of |x|^2 (a fingerprint) */
+} qcd_hip_result;
+int qcd_hip_serve(const qcd_hip_request *req, int n, int workers, size_t budget_bytes, int realtime,
+                  qcd_hip_result *res, double *wall_s, double *const *solutions);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,20 @@ class SolverParams(ctypes.Structure):
                 ("batch", ctypes.c_int)]
 
 
+class Request(ctypes.Structure):
+    """qcd_hip_request (include/qcd_hip.h)."""
+    _fields_ = [("arrival_s", ctypes.c_double), ("L", ctypes.c_int * 4), ("kappa", ctypes.c_double),
+                ("eps", ctypes.c_double), ("seed", ctypes.c_uint64), ("nrhs", ctypes.c_int),
+                ("inner_prec", ctypes.c_int), ("n_restart", ctypes.c_int), ("tol", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    """qcd_hip_result (include/qcd_hip.h)."""
+    _fields_ = [("queued_s", ctypes.c_double), ("service_s", ctypes.c_double), ("latency_s", ctypes.c_double),
+                ("rc", ctypes.c_int), ("outer", ctypes.c_int), ("iters", ctypes.c_int), ("relres", ctypes.c_double),
+                ("bytes", ctypes.c_double), ("xnorm2", ctypes.c_double)]
+
+
 class SolverStats(ctypes.Structure):
     """qcd_hip_solver_stats (include/qcd_hip.h)."""
     _fields_ = [("outer", ctypes.c_int), ("inner_iters", ctypes.c_int), ("relres", ctypes.c_double),
@@ -122,6 +136,11 @@ def lib():
         "qcd_hip_solver_solve": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverStats)]),
         "qcd_hip_invert": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverParams), ctypes.POINTER(SolverStats)]),
         "qcd_hip_b_minus_Dx": (i, [_vp, i, _vp, i, _vp, d, d, _dp]),
+        "qcd_hip_gauge_generate": (i, [_vp, d, ctypes.c_uint64]),
+        "qcd_hip_point_source": (i, [_vp, i, ctypes.POINTER(ctypes.c_int), i, i]),
+        "qcd_hip_release_cache": (None, []),
+        "qcd_hip_serve": (i, [ctypes.POINTER(Request), i, i, sz, i, ctypes.POINTER(Result), _dp,
+                              ctypes.POINTER(_dp)]),
         # drop-in reference API
         "qcd_initGeometry": (i, [pg, ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_uint16), _dp, i, i]),
         "qcd_destroyGeometry": (None, [pg]),
@@ -283,6 +302,11 @@ def gauge_generate(L, eps, seed):
     return u
 
 
+def gauge_generate_device(gauge64, eps, seed):
+    """The generator recipe run on the GPU into an FP64 gauge field."""
+    _check(lib().qcd_hip_gauge_generate(gauge64.h, float(eps), ctypes.c_uint64(seed)), "qcd_hip_gauge_generate")
+
+
 def spinor_gaussian(L, seed):
     V = int(np.prod(L))
     v = np.empty((V, 4, 3), dtype=np.complex128)
@@ -395,6 +419,40 @@ def b_minus_Dx(x, b, gauge64, kappa, mu=0.0, rx=0, rb=0):
     _check(lib().qcd_hip_b_minus_Dx(x.h, int(rx), b.h, int(rb), gauge64.h, float(kappa), float(mu), _p(rel)),
            "qcd_hip_b_minus_Dx")
     return float(rel[0])
+
+
+# ------------------------------------------------------------ serving
+def serve(requests, workers=4, budget_bytes=200 << 30, realtime=True, want_solutions=False):
+    """Run the serving loop over a list of request dicts (keys of qcd_hip_request;
+    inner_prec may be a string).  Returns (results list of dicts, wall seconds,
+    solutions or None)."""
+    n = len(requests)
+    arr = (Request * n)()
+    for k, r in enumerate(requests):
+        q = arr[k]
+        q.arrival_s = float(r.get("arrival_s", 0.0))
+        for d in range(4):
+            q.L[d] = int(r["L"][d])
+        q.kappa = float(r["kappa"])
+        q.eps = float(r["eps"])
+        q.seed = int(r["seed"])
+        q.nrhs = int(r.get("nrhs", 1))
+        ip = r.get("inner_prec", "f16")
+        q.inner_prec = PREC[ip] if isinstance(ip, str) else int(ip)
+        q.n_restart = int(r.get("n_restart", 32))
+        q.tol = float(r.get("tol", 1e-10))
+    res = (Result * n)()
+    wall = np.zeros(1)
+    sols = None
+    sol_ptrs = None
+    if want_solutions:
+        sols = [np.zeros((r.get("nrhs", 1), int(np.prod(r["L"])), 4, 3), dtype=np.complex128) for r in requests]
+        sol_ptrs = (_dp * n)(*[a.ctypes.data_as(_dp) for a in sols])
+    rc = lib().qcd_hip_serve(arr, n, int(workers), ctypes.c_size_t(int(budget_bytes)), int(bool(realtime)), res,
+                             _p(wall), sol_ptrs)
+    _check(rc, "qcd_hip_serve")
+    out = [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
+    return out, float(wall[0]), sols
 
 
 # ------------------------------------------------------------ drop-in helpers

@@ -167,6 +167,7 @@ using namespace qcdhip;
 extern "C" {
 
 int qcd_hip_norm2(qcd_hip_field v, int rhs, double *out) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch and stream
   QH_TRY(ensure_init());
   QH_TRY(check_rhs(v, rhs, "norm2"));
   Context *c = ctx();
@@ -183,6 +184,7 @@ int qcd_hip_norm2(qcd_hip_field v, int rhs, double *out) {
 }
 
 int qcd_hip_dot(qcd_hip_field a, int ra, qcd_hip_field b, int rb, double out[2]) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch and stream
   QH_TRY(ensure_init());
   QH_TRY(check_rhs(a, ra, "dot"));
   QH_TRY(check_rhs(b, rb, "dot"));
@@ -205,6 +207,7 @@ int qcd_hip_dot(qcd_hip_field a, int ra, qcd_hip_field b, int rb, double out[2])
 
 int qcd_hip_lincomb(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx, double bre,
                     double bim, qcd_hip_field y, int ry, int t0) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch and stream
   QH_TRY(ensure_init());
   QH_TRY(lincomb_launch(out, ro, are, aim, x, rx, bre, bim, y, ry, t0, ctx()->stream));
   QH_CHECK(hipStreamSynchronize(ctx()->stream));
@@ -212,6 +215,7 @@ int qcd_hip_lincomb(qcd_hip_field out, int ro, double are, double aim, qcd_hip_f
 }
 
 int qcd_hip_axpy(qcd_hip_field y, int ry, double are, double aim, qcd_hip_field x, int rx) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch and stream
   QH_TRY(ensure_init());
   QH_TRY(check_rhs(y, ry, "axpy"));
   QH_TRY(check_rhs(x, rx, "axpy"));

@@ -279,10 +279,15 @@ struct TuneVal {
 static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 static int g_autotune = -1;   // -1: read QCD_HIP_AUTOTUNE once
 
-static const TuneVal *tune_lookup(const TuneKey &k) {
+// copy of the tuned entry for k (the table may grow under other threads)
+static bool tune_get(const TuneKey &k, TuneVal *out) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   for (auto &e : g_tune)
-    if (e.first == k) return &e.second;
-  return nullptr;
+    if (e.first == k) {
+      if (out) *out = e.second;
+      return true;
+    }
+  return false;
 }
 
 static TuneKey key_of(qcd_hip_field in, qcd_hip_field g) {
@@ -335,8 +340,9 @@ static const int kMaps[] = {MAP_ZMARCH | (1 << 8), MAP_ZMARCH | (2 << 8), MAP_ZM
 static const int kOccs[] = {0, 2, 3, 16};
 
 int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   const TuneKey key = key_of(in, g);
-  if (!force && tune_lookup(key)) return 0;
+  if (!force && tune_get(key, nullptr)) return 0;
   hipStream_t st = ctx()->stream;
   hipEvent_t e0, e1;
   QH_CHECK(hipEventCreate(&e0));
@@ -414,8 +420,9 @@ int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
 // the tuned configuration of this (lattice, precision) if there is one.
 int dslash_launch(int epi, bool red, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
                   qcd_hip_field b, int rb, double m, double tm, double *partials, hipStream_t stream, int *grid_out) {
-  const TuneVal *t = tune_lookup(key_of(in, g));
-  const int map = t ? t->map : kDefaultMap, occ = t ? t->occ : 0;
+  TuneVal t;
+  const bool have = tune_get(key_of(in, g), &t);
+  const int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out);
 }
 
@@ -423,8 +430,9 @@ int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs
                         qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m,
                         double tm, double *partials, hipStream_t stream, int *grid_out) {
   if (count < 1) return 0;
-  const TuneVal *t = tune_lookup(key_of(in, g));
-  const int map = t ? t->map : kDefaultMap, occ = t ? t->occ : 0;
+  TuneVal t;
+  const bool have = tune_get(key_of(in, g), &t);
+  const int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
   const Batch bt{ids, count, in_rs, out_rs, b_rs};
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out, bt);
 }
@@ -446,9 +454,10 @@ int dslash_grid_bound(qcd_hip_lattice lat) {
 }
 
 int dslash_grid(qcd_hip_field in, qcd_hip_field g) {
-  const TuneVal *t = tune_lookup(key_of(in, g));
+  TuneVal t;
+  const bool have = tune_get(key_of(in, g), &t);
   DslashArgs a = make_args(in->lat, 0.0, 0.0);
-  return set_traversal(a, t ? t->map : kDefaultMap);
+  return set_traversal(a, have ? t.map : kDefaultMap);
 }
 
 }  // namespace qcdhip
@@ -509,12 +518,13 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
   QH_TRY(ensure_init());
   QH_TRY(dslash_check(out, in, gauge));
   QH_TRY(dslash_tune(out, in, gauge, 1));
-  const TuneVal *t = tune_lookup(key_of(in, gauge));
+  TuneVal t{0, 0, 0.f};
+  tune_get(key_of(in, gauge), &t);
   if (tuned) {
-    tuned[0] = t->map;
-    tuned[1] = t->occ;
+    tuned[0] = t.map;
+    tuned[1] = t.occ;
   }
-  if (ms) *ms = t->ms;
+  if (ms) *ms = t.ms;
   return 0;
 }
 
@@ -528,6 +538,7 @@ int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, in
     return 1;
   }
   const TuneKey k = key_of(in, gauge);
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   for (auto &e : g_tune)
     if (e.first == k) {
       e.second = TuneVal{map, occ, 0.f};

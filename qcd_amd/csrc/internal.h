@@ -16,6 +16,8 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <cfloat>
 #include <cstdint>
 #include <cstdio>
@@ -53,7 +55,7 @@ struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
   int num_cu = 0;
-  size_t bytes_in_use = 0;            // device bytes held by library objects
+  std::atomic<size_t> bytes_in_use{0}; // device bytes held by library objects
   // reduction scratch shared by the BLAS of this context
   double *partials = nullptr;         // [kMaxReduceBlocks * 2 * (kMaxKrylov + 4)]
   double *scalars = nullptr;          // small device scalar area
@@ -61,6 +63,9 @@ struct Context {
 };
 
 Context *ctx();                       // current context (initialised on first use)
+// Serialises the library's shared host-side state (staging buffer, scratch
+// fields, tuning table) when several host threads drive separate streams.
+std::recursive_mutex &state_mutex();
 int ensure_init();
 int dev_alloc(void **p, size_t bytes);
 void dev_free(void *p, size_t bytes);

@@ -1,10 +1,12 @@
 // runtime.hip -- context, errors, lattices, fields, host<->device transfer,
 // precision conversion and the synthetic-input generator.
 #include "internal.h"
+#include "generator.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <map>
 #include <thread>
 
 namespace qcdhip {
@@ -26,19 +28,48 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 
 Context *ctx() { return &g_ctx; }
 
+std::recursive_mutex &state_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 int ensure_init() {
   if (g_ctx.stream) return 0;
   return qcd_hip_init(-1) ? 3 : 0;
 }
 
+// Caching allocator: freed blocks are kept (up to kPoolCap bytes) and reused
+// for requests of the same size.  hipFree synchronises the whole device, so
+// freeing inside a serving loop would serialise concurrent requests.
+constexpr size_t kPoolCap = (size_t)64 << 30;
+static std::multimap<size_t, void *> g_pool;
+static size_t g_pool_bytes = 0;
+
 int dev_alloc(void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return 0;
+  {
+    std::lock_guard<std::recursive_mutex> lock(state_mutex());
+    auto it = g_pool.find(bytes);
+    if (it != g_pool.end()) {
+      *p = it->second;
+      g_pool.erase(it);
+      g_pool_bytes -= bytes;
+      g_ctx.bytes_in_use += bytes;
+      return 0;
+    }
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
+    // give the cached blocks back to the runtime and try once more
+    qcd_hip_release_cache();
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
     set_error("out of device memory: requested %zu bytes with %zu held by the library", bytes,
-              g_ctx.bytes_in_use);
+              g_ctx.bytes_in_use.load());
     *p = nullptr;
     return 4;
   }
@@ -48,8 +79,16 @@ int dev_alloc(void **p, size_t bytes) {
 
 void dev_free(void *p, size_t bytes) {
   if (!p) return;
-  (void)hipFree(p);
   g_ctx.bytes_in_use -= bytes;
+  {
+    std::lock_guard<std::recursive_mutex> lock(state_mutex());
+    if (g_pool_bytes + bytes <= kPoolCap) {
+      g_pool.emplace(bytes, p);
+      g_pool_bytes += bytes;
+      return;
+    }
+  }
+  (void)hipFree(p);
 }
 
 // Device staging buffer for AoS transfers, grown on demand.
@@ -75,6 +114,7 @@ static int stage(size_t bytes, void **out) {
 static qcd_hip_field g_scratch[8] = {nullptr};
 
 int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_field *out) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   if (slot < 0 || slot >= 8) {
     set_error("scratch slot %d out of range", slot);
     return 1;
@@ -266,7 +306,14 @@ int qcd_hip_synchronize(void) {
   return 0;
 }
 
-size_t qcd_hip_bytes_in_use(void) { return g_ctx.bytes_in_use; }
+size_t qcd_hip_bytes_in_use(void) { return g_ctx.bytes_in_use.load(); }
+
+void qcd_hip_release_cache(void) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  for (auto &b : g_pool) (void)hipFree(b.second);
+  g_pool.clear();
+  g_pool_bytes = 0;
+}
 
 // ------------------------------------------------------------ lattices
 int qcd_hip_lattice_create(qcd_hip_lattice *out, const int L[4], const double theta[4]) {
@@ -335,7 +382,10 @@ static int field_alloc(qcd_hip_field *out, qcd_hip_lattice lat, int kind, int pr
   }
   f->norm = norm_bytes ? (float *)((char *)f->data + data_bytes) : nullptr;
   f->bytes = data_bytes + norm_bytes;
+  // complete before returning: the field may next be used on any stream, and
+  // the null stream does not order against the library's non-blocking streams
   hipError_t e = hipMemsetAsync(f->data, 0, f->bytes, g_ctx.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
   if (e != hipSuccess) {
     dev_free(f->data, f->bytes);
     delete f;
@@ -365,6 +415,7 @@ static void *rhs_ptr(qcd_hip_field f, int rhs) {
 }
 
 int qcd_hip_spinor_upload(qcd_hip_field f, int rhs, const double *aos) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   QH_TRY(ensure_init());
   if (!f || f->kind != QCD_HIP_SPINOR || rhs < 0 || rhs >= f->nrhs || !aos) {
     set_error("spinor upload: bad field, rhs or pointer");
@@ -388,6 +439,7 @@ int qcd_hip_spinor_upload(qcd_hip_field f, int rhs, const double *aos) {
 }
 
 int qcd_hip_spinor_download(qcd_hip_field f, int rhs, double *aos) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   QH_TRY(ensure_init());
   if (!f || f->kind != QCD_HIP_SPINOR || rhs < 0 || rhs >= f->nrhs || !aos) {
     set_error("spinor download: bad field, rhs or pointer");
@@ -411,6 +463,7 @@ int qcd_hip_spinor_download(qcd_hip_field f, int rhs, double *aos) {
 }
 
 int qcd_hip_gauge_upload(qcd_hip_field f, const double *aos) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   QH_TRY(ensure_init());
   if (!f || f->kind != QCD_HIP_GAUGE || !aos) {
     set_error("gauge upload: bad field or pointer");
@@ -433,6 +486,7 @@ int qcd_hip_gauge_upload(qcd_hip_field f, const double *aos) {
 }
 
 int qcd_hip_gauge_download(qcd_hip_field f, double *aos) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
   QH_TRY(ensure_init());
   if (!f || f->kind != QCD_HIP_GAUGE || !aos) {
     set_error("gauge download: bad field or pointer");
@@ -485,129 +539,26 @@ int qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src) {
     return 1;
   }
   const long V = src->lat->V, st = src->lat->stride;
-  if (src->kind == QCD_HIP_GAUGE) return convert_planes<36>(src->prec, src->data, nullptr, dst->prec, dst->data, nullptr, V, st);
-  for (int r = 0; r < src->nrhs; r++) {
-    const float *sn = src->norm ? src->norm + (size_t)r * st : nullptr;
-    float *dn = dst->norm ? dst->norm + (size_t)r * st : nullptr;
-    QH_TRY(convert_planes<12>(src->prec, rhs_ptr(src, r), sn, dst->prec, rhs_ptr(dst, r), dn, V, st));
+  if (src->kind == QCD_HIP_GAUGE) {
+    QH_TRY(convert_planes<36>(src->prec, src->data, nullptr, dst->prec, dst->data, nullptr, V, st));
+  } else {
+    for (int r = 0; r < src->nrhs; r++) {
+      const float *sn = src->norm ? src->norm + (size_t)r * st : nullptr;
+      float *dn = dst->norm ? dst->norm + (size_t)r * st : nullptr;
+      QH_TRY(convert_planes<12>(src->prec, rhs_ptr(src, r), sn, dst->prec, rhs_ptr(dst, r), dn, V, st));
+    }
   }
+  QH_CHECK(hipStreamSynchronize(g_ctx.stream));   // the copy may be used on another stream next
   return 0;
 }
 
 }  // extern "C"
 
 // ================================================================ generator
-// Counter-based draws: draw k of stream s under seed is
-//   mix(key + k * 0x632BE59BD9B4E019), key = mix(seed) ^ s * 0xD1B54A32D192ED03,
-// mix = the splitmix64 finaliser; uniforms in (0,1) from the top 53 bits;
-// normals by Box-Muller.  Links U = reunitarize(exp(i eps H)),
-// H = (A + A^+)/2 made traceless with A_ij = N(0,1) + i N(0,1); exp by a
-// degree-16 Taylor polynomial of X / 2^s (|X/2^s|_inf <= 1/4) and s squarings;
-// reunitarisation by Gram-Schmidt of rows 0 and 1, row 2 = conj(row0 x row1).
+// The recipe lives in generator.h (shared with the device kernels below).
 namespace {
 
-inline uint64_t mix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ULL;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-inline double unit_draw(uint64_t key, uint64_t k) {
-  uint64_t r = mix64(key + k * 0x632BE59BD9B4E019ULL);
-  return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-void gaussians(double *out, int n, uint64_t seed, uint64_t stream) {
-  const uint64_t key = mix64(seed) ^ (stream * 0xD1B54A32D192ED03ULL);
-  for (int i = 0; i < n; i += 2) {
-    double u1 = unit_draw(key, (uint64_t)i), u2 = unit_draw(key, (uint64_t)i + 1);
-    double r = std::sqrt(-2.0 * std::log(u1));
-    out[i] = r * std::cos(6.283185307179586 * u2);
-    out[i + 1] = r * std::sin(6.283185307179586 * u2);
-  }
-}
-
-struct C {
-  double re, im;
-};
-inline C cm(C a, C b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-inline C ca(C a, C b) { return {a.re + b.re, a.im + b.im}; }
-
-void m3mul(C r[3][3], const C a[3][3], const C b[3][3]) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      C s = {0.0, 0.0};
-      for (int k = 0; k < 3; k++) s = ca(s, cm(a[i][k], b[k][j]));
-      r[i][j] = s;
-    }
-}
-
-void su3_link(C U[3][3], const double g[18], double eps) {
-  C H[3][3], X[3][3], E[3][3], T[3][3];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      H[i][j].re = 0.5 * (g[2 * (3 * i + j)] + g[2 * (3 * j + i)]);
-      H[i][j].im = 0.5 * (g[2 * (3 * i + j) + 1] - g[2 * (3 * j + i) + 1]);
-    }
-  double tr = (H[0][0].re + H[1][1].re + H[2][2].re) / 3.0;
-  for (int i = 0; i < 3; i++) {
-    H[i][i].re -= tr;
-    H[i][i].im = 0.0;
-  }
-  double nrm = 0.0, scale = 1.0;
-  int s = 0;
-  for (int i = 0; i < 3; i++) {
-    double row = 0.0;
-    for (int j = 0; j < 3; j++) {
-      X[i][j].re = -eps * H[i][j].im;
-      X[i][j].im = eps * H[i][j].re;
-      row += std::fabs(X[i][j].re) + std::fabs(X[i][j].im);
-    }
-    nrm = std::max(nrm, row);
-  }
-  while (nrm * scale > 0.25) {
-    scale *= 0.5;
-    s++;
-  }
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      X[i][j].re *= scale;
-      X[i][j].im *= scale;
-    }
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) E[i][j] = {i == j ? 1.0 : 0.0, 0.0};
-  for (int k = 16; k >= 1; k--) {
-    m3mul(T, X, E);
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        E[i][j].re = T[i][j].re / (double)k + (i == j ? 1.0 : 0.0);
-        E[i][j].im = T[i][j].im / (double)k;
-      }
-  }
-  for (int k = 0; k < s; k++) {
-    m3mul(T, E, E);
-    std::memcpy(E, T, sizeof(E));
-  }
-  double n0 = 0.0, n1 = 0.0;
-  for (int j = 0; j < 3; j++) n0 += E[0][j].re * E[0][j].re + E[0][j].im * E[0][j].im;
-  n0 = std::sqrt(n0);
-  for (int j = 0; j < 3; j++) U[0][j] = {E[0][j].re / n0, E[0][j].im / n0};
-  C p = {0.0, 0.0};
-  for (int j = 0; j < 3; j++) p = ca(p, cm({U[0][j].re, -U[0][j].im}, E[1][j]));
-  for (int j = 0; j < 3; j++) {
-    C q = cm(p, U[0][j]);
-    U[1][j] = {E[1][j].re - q.re, E[1][j].im - q.im};
-  }
-  for (int j = 0; j < 3; j++) n1 += U[1][j].re * U[1][j].re + U[1][j].im * U[1][j].im;
-  n1 = std::sqrt(n1);
-  for (int j = 0; j < 3; j++) U[1][j] = {U[1][j].re / n1, U[1][j].im / n1};
-  for (int j = 0; j < 3; j++) {
-    int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-    C x = cm(U[0][j1], U[1][j2]), y = cm(U[0][j2], U[1][j1]);
-    U[2][j] = {x.re - y.re, -(x.im - y.im)};
-  }
-}
+using qcdgen::C;
 
 template <typename F>
 void parallel_for(long n, F f) {
@@ -625,7 +576,58 @@ void parallel_for(long n, F f) {
   for (auto &x : th) x.join();
 }
 
+// one lane per link (site, mu), written straight into the tiled F64 layout
+__global__ void __launch_bounds__(kBlock) k_gauge_generate(double2 *data, long V, long stride, double eps,
+                                                           uint64_t seed) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * V) return;
+  double g[18];
+  qcdgen::gaussians(g, 18, seed, (uint64_t)i);
+  C U[3][3];
+  qcdgen::su3_link(U, g, eps);
+  const long site = i / 4;
+  const int mu = (int)(i % 4);
+  for (int k = 0; k < 9; k++)
+    data[gauge_index(9 * mu + k, (size_t)site, (size_t)stride)] = make_double2(U[k / 3][k % 3].re, U[k / 3][k % 3].im);
+}
+
+// field rhs = delta(site) on component (spin, col), zero elsewhere
+__global__ void __launch_bounds__(kBlock) k_point_source(double2 *rhs0, long V, long site, int comp) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  for (int k = 0; k < 12; k++)
+    rhs0[tile_index(12, k, (size_t)s)] = make_double2(s == site && k == comp ? 1.0 : 0.0, 0.0);
+}
+
 }  // namespace
+
+namespace qcdhip {
+
+int gauge_generate_device(qcd_hip_field g, double eps, uint64_t seed, hipStream_t st) {
+  if (!g || g->kind != QCD_HIP_GAUGE || g->prec != QCD_HIP_F64) {
+    set_error("gauge_generate: need an FP64 gauge field");
+    return 1;
+  }
+  const long V = g->lat->V;
+  k_gauge_generate<<<grid_for(4 * V), kBlock, 0, st>>>((double2 *)g->data, V, g->lat->stride, eps, seed);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int point_source_device(qcd_hip_field f, int rhs, long site, int spin, int col, hipStream_t st) {
+  if (!f || f->kind != QCD_HIP_SPINOR || f->prec != QCD_HIP_F64 || rhs < 0 || rhs >= f->nrhs || site < 0 ||
+      site >= f->lat->V || spin < 0 || spin > 3 || col < 0 || col > 2) {
+    set_error("point_source: bad field, rhs, site or component");
+    return 1;
+  }
+  const long V = f->lat->V;
+  k_point_source<<<grid_for(V), kBlock, 0, st>>>((double2 *)f->data + (size_t)rhs * 12 * f->lat->stride, V, site,
+                                                 3 * spin + col);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcdhip
 
 extern "C" {
 
@@ -636,8 +638,8 @@ int qcd_hip_gauge_generate_host(double *aos, long V, double eps, uint64_t seed) 
   }
   parallel_for(4 * V, [=](long i) {
     double g[18];
-    gaussians(g, 18, seed, (uint64_t)i);
-    su3_link(reinterpret_cast<C(*)[3]>(aos + 18 * i), g, eps);
+    qcdgen::gaussians(g, 18, seed, (uint64_t)i);
+    qcdgen::su3_link(reinterpret_cast<C(*)[3]>(aos + 18 * i), g, eps);
   });
   return 0;
 }
@@ -647,7 +649,32 @@ int qcd_hip_spinor_gaussian_host(double *aos, long V, uint64_t seed) {
     qcdhip::set_error("spinor generate: bad pointer or volume");
     return 1;
   }
-  parallel_for(V, [=](long i) { gaussians(aos + 24 * i, 24, seed ^ 0x5350494E4F52ULL, (uint64_t)i); });
+  parallel_for(V, [=](long i) { qcdgen::gaussians(aos + 24 * i, 24, seed ^ qcdgen::kSpinorSeedSalt, (uint64_t)i); });
+  return 0;
+}
+
+int qcd_hip_gauge_generate(qcd_hip_field g, double eps, uint64_t seed) {
+  QH_TRY(qcdhip::ensure_init());
+  QH_TRY(qcdhip::gauge_generate_device(g, eps, seed, qcdhip::ctx()->stream));
+  QH_CHECK(hipStreamSynchronize(qcdhip::ctx()->stream));
+  return 0;
+}
+
+int qcd_hip_point_source(qcd_hip_field f, int rhs, const int site[4], int spin, int col) {
+  QH_TRY(qcdhip::ensure_init());
+  if (!f || !site) {
+    qcdhip::set_error("point_source: null argument");
+    return 1;
+  }
+  const int *L = f->lat->L;
+  for (int d = 0; d < 4; d++)
+    if (site[d] < 0 || site[d] >= L[d]) {
+      qcdhip::set_error("point_source: coordinate %d out of range", d);
+      return 1;
+    }
+  const long l = site[0] + (long)L[0] * (site[1] + (long)L[1] * (site[2] + (long)L[2] * site[3]));
+  QH_TRY(qcdhip::point_source_device(f, rhs, l, spin, col, qcdhip::ctx()->stream));
+  QH_CHECK(hipStreamSynchronize(qcdhip::ctx()->stream));
   return 0;
 }
 

@@ -92,6 +92,7 @@ using namespace qcdhip;
 extern "C" {
 
 int qcd_hip_gauss_smear(qcd_hip_field v, qcd_hip_field gauge, double alpha, int t, int niter) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch field and stream
   QH_TRY(ensure_init());
   if (!v || !gauge || v->kind != QCD_HIP_SPINOR || gauge->kind != QCD_HIP_GAUGE || v->lat->V != gauge->lat->V) {
     qcdhip::set_error("gauss_smear: expected a spinor and a gauge field on one lattice");

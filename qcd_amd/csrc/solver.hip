@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(1024) k_scal_rnorm(GcrScal *scs, const double 
     GcrScal *sc = scs + b;
     sc->rnorm = sqrt(sums[0]);
     sc->rscale = sc->rnorm > 0.0 ? 1.0 / sc->rnorm : 0.0;
-    rn[b] = sc->rnorm;
+    rn[b] = sc->bad ? __builtin_nan("") : sc->rnorm;   // a breakdown reaches the host as NaN
   }
 }
 
@@ -351,6 +351,7 @@ struct qcd_hip_solver_s {
   qcd_hip_lattice lat;
   qcd_hip_solver_params prm;
   int B = 1;                        // right-hand sides per batch
+  hipStream_t stream = nullptr;     // the solver's own stream
   qcd_hip_field g64 = nullptr;      // caller's FP64 gauge field (not owned)
   qcd_hip_field gin = nullptr;      // inner-precision copy (owned unless F64)
   qcd_hip_field Rs = nullptr;       // n*B search directions res_j (inner precision)
@@ -466,16 +467,7 @@ int solve_batch(qcd_hip_solver_s *S, qcd_hip_field x, qcd_hip_field b, int x0, i
       rel[i] = S->rn_host[i] / bnorm[i];
       if (!(rel[i] == rel[i])) bad = 1;
     }
-    int flag = 0;
-    for (int i : act) {
-      int f = 0;
-      QH_CHECK(hipMemcpy(&f, &S->sc[i].bad, sizeof(int), hipMemcpyDeviceToHost));
-      flag |= f;
-    }
-    if (flag || bad) {
-      bad = 1;
-      break;
-    }
+    if (bad) break;
   }
   for (int i = 0; i < cnt; i++) {
     tot->outer += outer[i];
@@ -514,6 +506,11 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   auto *S = new qcd_hip_solver_s();
   S->lat = lat;
   S->prm = *prm;
+  if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete S;
+    set_error("solver_create: cannot create a stream");
+    return 2;
+  }
   S->B = prm->batch;
   S->nb = (unsigned)grid_for(lat->V);
   const int n = prm->n_restart, B = S->B;
@@ -551,7 +548,10 @@ void qcd_hip_solver_destroy(qcd_hip_solver S) {
   qcd_hip_field_destroy(S->r64);
   qcd_hip_field_destroy(S->bq);
   if (S->gin && S->gin != S->g64) qcd_hip_field_destroy(S->gin);
-  (void)hipStreamSynchronize(ctx()->stream);
+  if (S->stream) {
+    (void)hipStreamSynchronize(S->stream);
+    (void)hipStreamDestroy(S->stream);
+  }
   dev_free(S->sc, sizeof(GcrScal) * S->B);
   dev_free(S->rn, sizeof(double) * S->B);
   dev_free(S->ids, sizeof(int) * S->B);
@@ -601,7 +601,7 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
     set_error("solver_solve: kappa must be positive");
     return 1;
   }
-  hipStream_t st = ctx()->stream;
+  hipStream_t st = S->stream;
   const auto t0 = std::chrono::steady_clock::now();
   // Tune the inner and outer operators before the first timed use.
   QH_TRY(dslash_prepare(S->Ws, S->Rs, S->gin));
@@ -624,6 +624,7 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
 // ||D x - b|| / ||b|| in FP64 with m = 1/(2 kappa) - 4 (b_minus_Dx.c:152-167)
 int qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field g, double kappa, double mu,
                        double *rel) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch field and stream
   QH_TRY(ensure_init());
   if (!x || !b || !g || x->prec != QCD_HIP_F64 || b->prec != QCD_HIP_F64 || g->prec != QCD_HIP_F64 || !rel ||
       rx < 0 || rx >= x->nrhs || rb < 0 || rb >= b->nrhs) {
