@@ -1,10 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark: Wilson Dslash HBM GB/s (headline), solves/s and latency legs.
+"""Benchmark of the Wilson Dslash / propagator-solve hot path on one MI355X.
 
-One "step" is one application of the FP64 Wilson Dslash (hopping stencil +
-mass term) to one spinor on a 24^3x48 lattice, inputs resident in HBM.
-Algorithmic bytes per site: spinor in 192 + links 576 + spinor out 192 =
-960 B (FP64).  Prints one JSON line (see the README of this repository).
+Headline (`value`): FP64 Wilson Dslash (hopping stencil + mass term) on a
+24^3x48 lattice, one right-hand side, inputs resident in HBM, in algorithmic
+HBM GB/s: 960 B per site (spinor in 192 + 4 links 576 + spinor out 192) over
+the mean kernel time measured with HIP events on the library stream.  One
+"step" is one such application.
+
+Further legs reported in the same JSON line:
+  dslash   FP32 (480 B/site) and FP16-storage (248 B/site) on 24^3x48, FP64 on 48^3x96
+  solves   16^3x32, kappa 0.130, eps 0.3, 12 point sources at the origin:
+           plain FP64 GCR solver one rhs at a time (invert.c's structure) vs
+           mixed FP32 / FP16 inner with FP64 outer, every rhs to
+           ||b - D x|| / ||b|| <= 1e-10 checked in FP64
+  serve    a Poisson stream of propagator-solve requests (volumes, kappa, eps,
+           rhs counts drawn per request) served continuously, mixed vs FP64:
+           solves/s and p50 / p99 latency
+  cpu_baseline  the CPU oracle (C restatement of the reference) on the host cores
 """
 import argparse
 import json
@@ -38,13 +50,12 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     """Mean Dslash time (HIP events on the library stream) and GB/s."""
     V = int(np.prod(L))
     lat = Q.Lattice(L)
-    u = Q.gauge_generate(L, 0.3, seed)
+    g64 = Q.Gauge(lat)
+    Q.gauge_generate_device(g64, 0.3, seed)
     psi = Q.spinor_gaussian(L, seed + 1)
-    g64 = Q.Gauge(lat).upload(u)
     p = Q.PREC[prec]
-    if p == Q.F64:
-        g = g64
-    else:
+    g = g64
+    if p != Q.F64:
         g = Q.Gauge(lat, p)
         Q.convert(g, g64)
     x = Q.Spinor(lat, p).upload(psi)
@@ -52,12 +63,61 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     m = 1 / (2 * 0.132) - 4
     cfg = Q.dslash_tune(y, x, g)
     Q.synchronize()
-    t0 = time.perf_counter()
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
-    wall = time.perf_counter() - t0
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
-    return {"prec": prec, "L": list(L), "ms": ms, "GBps": gbps, "wall_s": wall,
-            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab"][cfg[0]], "min_waves": cfg[1]}}
+    return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
+            "config": {"traversal": "z-march" if (cfg[0] & 0xff) == 2 else ["xcd-contiguous", "xcd-y-slab"][cfg[0]],
+                       "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
+                       "sched_fence": bool(cfg[1] & 16)}}
+
+
+def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0.3, seed=2002):
+    """Config-2 propagator: 12 point sources at the origin, FP64-verified."""
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat)
+    Q.gauge_generate_device(g, eps, seed)
+    b = Q.Spinor(lat, nrhs=12)
+    for r in range(12):
+        Q.point_source_device(b, r, (0, 0, 0, 0), r // 3, r % 3)
+    x = Q.Spinor(lat, nrhs=12)
+    S = Q.Solver(lat, g, n_restart=n_restart, tol=1e-10, inner_prec=prec, batch=batch, max_outer=5000)
+    b1 = Q.Spinor(lat).upload(Q.point_source(L, (0, 0, 0, 0), 0, 0))
+    x1 = Q.Spinor(lat)
+    S1 = Q.Solver(lat, g, n_restart=n_restart, tol=1e-4, inner_prec=prec, batch=1)
+    S1.solve(x1, b1, kappa)   # tunes the operators outside the timed region
+    Q.synchronize()
+    t = time.perf_counter()
+    st = S.solve(x, b, kappa)
+    Q.synchronize()
+    el = time.perf_counter() - t
+    worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
+    assert worst <= 1e-10, f"{prec} solve not converged: {worst}"
+    return {"solves_per_s": 12 / el, "seconds": el, "rhs": 12, "batch": batch, "n_restart": n_restart,
+            "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
+
+
+def serve_stream(n, rate, seed):
+    """Poisson arrivals; volumes, kappa, eps and rhs counts drawn per request."""
+    rng = np.random.default_rng(seed)
+    vols = [(16, 8, 8, 8), (32, 16, 16, 16), (48, 24, 24, 24)]
+    t, reqs = 0.0, []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        reqs.append({"arrival_s": t, "L": vols[rng.integers(3)], "kappa": float(rng.uniform(0.120, 0.132)),
+                     "eps": float(rng.uniform(0.2, 0.35)), "seed": int(10000 + k),
+                     "nrhs": int(rng.choice([1, 12])), "n_restart": 32, "tol": 1e-10})
+    return reqs
+
+
+def serve_leg(Q, reqs, prec, workers=4, budget=200 << 30):
+    res, wall, _ = Q.serve([dict(r, inner_prec=prec) for r in reqs], workers=workers, budget_bytes=budget,
+                           realtime=True)
+    lat = np.array([r["latency_s"] for r in res])
+    solves = sum(r["nrhs"] for r in reqs)
+    return {"solves_per_s": solves / wall, "requests": len(reqs), "solves": solves, "wall_s": wall,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_queue_s": float(np.mean([r["queued_s"] for r in res])),
+            "worst_relres": max(r["relres"] for r in res)}
 
 
 def cpu_baseline_leg(L, seconds=10.0):
@@ -75,12 +135,28 @@ def cpu_baseline_leg(L, seconds=10.0):
         O.dslash(geo, psi, u, m)
         n += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 200:
+        if el >= seconds or n >= 400:
             break
     V = int(np.prod(L))
     return {"value": V * 960 / (el / n) / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{n} FP64 Dslash applies of the CPU oracle (OpenMP, {threads} threads) on "
-                      f"{L[1]}^3x{L[0]} in {el:.1f} s, 960 B/site"}
+            "sample": f"{n} FP64 Dslash applies of the CPU oracle (C, OpenMP over sites, {threads} threads) on "
+                      f"{L[1]}^3x{L[0]} in {el:.1f} s, 960 B/site; same workload as the headline"}
+
+
+def cpu_solver_sample(cycles_per_rhs, n_restart=32):
+    """One GCR(32) cycle of the oracle (invert.c restated, sequential BLAS as
+    qcd_blas.c) at 16^3x32, scaled to solves/s with the GPU's cycle count."""
+    import oracle as O
+    L = (32, 16, 16, 16)
+    O.set_threads(cpu_threads())
+    u = O.gauge_generate(L, 0.3, 2002)
+    b = O.point_source(L, (0, 0, 0, 0), 0, 0)
+    geo = O.Geometry(L)
+    t = time.perf_counter()
+    O.gcr(geo, b, u, 0.130, 0.0, n=n_restart, op=0, keep_q1=True)
+    el = time.perf_counter() - t
+    return {"solves_per_s": 1.0 / (el * cycles_per_rhs), "seconds_per_gcr_cycle": el,
+            "cycles_per_rhs_assumed": cycles_per_rhs}
 
 
 def main():
@@ -88,7 +164,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-solve", action="store_true", help="skip the solver legs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one rank on one GPU; --gpus {args.gpus} is not supported "
@@ -105,23 +185,47 @@ def main():
 
     L = (48, 24, 24, 24)
     head = dslash_leg(Q, L, "f64", args.steps, args.warmup)
-    log(f"f64 dslash 24^3x48: {head['ms']*1e3:.1f} us, {head['GBps']:.0f} GB/s")
-    extra = {}
+    log(f"f64 dslash 24^3x48: {head['ms']*1e3:.1f} us, {head['GBps']:.0f} GB/s {head['config']}")
+    dsl = {"f64": head}
     for prec in ("f32", "f16"):
-        r = dslash_leg(Q, L, prec, args.steps, args.warmup)
-        extra[prec] = {"ms": r["ms"], "GBps": r["GBps"], "frac": r["GBps"] / HBM_PEAK_GBPS, "config": r["config"]}
-        log(f"{prec} dslash 24^3x48: {r['ms']*1e3:.1f} us, {r['GBps']:.0f} GB/s")
-    L5 = (96, 48, 48, 48)
-    big = dslash_leg(Q, L5, "f64", max(10, args.steps // 10), max(2, args.warmup // 10))
+        dsl[prec] = dslash_leg(Q, L, prec, args.steps, args.warmup)
+        log(f"{prec} dslash 24^3x48: {dsl[prec]['ms']*1e3:.1f} us, {dsl[prec]['GBps']:.0f} GB/s")
+    big = dslash_leg(Q, (96, 48, 48, 48), "f64", max(10, args.steps // 10), max(2, args.warmup // 10))
+    dsl["f64_48c96"] = big
     log(f"f64 dslash 48^3x96: {big['ms']*1e3:.1f} us, {big['GBps']:.0f} GB/s")
-    extra["f64_48c96"] = {"ms": big["ms"], "GBps": big["GBps"], "frac": big["GBps"] / HBM_PEAK_GBPS,
-                          "config": big["config"]}
-    extra["f64"] = {"config": head["config"]}
+
+    solves = None
+    if not args.no_solve:
+        solves = {"config": "16^3x32, kappa 0.130, eps 0.3, 12 point sources at the origin, tol 1e-10 (FP64)"}
+        solves["fp64_one_at_a_time"] = solves_leg(Q, "f64", 1, 32)
+        solves["mixed_f32_one_at_a_time"] = solves_leg(Q, "f32", 1, 32)
+        solves["mixed_f16_batched12"] = solves_leg(Q, "f16", 12, 32)
+        for k in ("fp64_one_at_a_time", "mixed_f32_one_at_a_time", "mixed_f16_batched12"):
+            log(f"solves {k}: {solves[k]['solves_per_s']:.2f}/s ({solves[k]['gcr_iterations']} GCR iterations)")
+        base = solves["fp64_one_at_a_time"]["solves_per_s"]
+        solves["speedup_mixed_f32_vs_fp64"] = solves["mixed_f32_one_at_a_time"]["solves_per_s"] / base
+        solves["speedup_mixed_f16_batched_vs_fp64"] = solves["mixed_f16_batched12"]["solves_per_s"] / base
+
+    serve = None
+    if not args.no_serve:
+        reqs = serve_stream(args.serve_requests, args.serve_rate, 777)
+        serve = {"stream": f"{args.serve_requests} requests, Poisson rate {args.serve_rate}/s, volumes "
+                           "{8^3x16,16^3x32,24^3x48}, kappa U[0.120,0.132], eps U[0.2,0.35], rhs {1,12}, "
+                           "4 workers, 200 GB budget, tol 1e-10"}
+        for prec in ("f16", "f64"):
+            serve["mixed_f16" if prec == "f16" else "fp64"] = serve_leg(Q, reqs, prec)
+            s = serve["mixed_f16" if prec == "f16" else "fp64"]
+            log(f"serve {prec}: {s['solves_per_s']:.2f} solves/s p50 {s['p50_latency_s']:.2f}s "
+                f"p99 {s['p99_latency_s']:.2f}s wall {s['wall_s']:.1f}s")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_leg(L)
         log(f"cpu baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})")
+        if solves:
+            cyc = solves["fp64_one_at_a_time"]["outer_cycles"] / 12.0
+            cpu["solver_sample"] = cpu_solver_sample(cyc)
+            log(f"cpu solver sample: {cpu['solver_sample']}")
 
     V = int(np.prod(L))
     out = {
@@ -143,7 +247,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(head["GBps"] / HBM_PEAK_GBPS, 4), "traffic": None},
         "cpu_baseline": cpu,
-        "dslash": extra,
+        "dslash": dsl,
+        "solves": solves,
+        "serve": serve,
         "device": dev,
     }
     print(json.dumps(out), flush=True)

@@ -307,6 +307,12 @@ def gauge_generate_device(gauge64, eps, seed):
     _check(lib().qcd_hip_gauge_generate(gauge64.h, float(eps), ctypes.c_uint64(seed)), "qcd_hip_gauge_generate")
 
 
+def point_source_device(f, rhs, site, spin, col):
+    """rhs of an FP64 device spinor = delta at site (t, x, y, z) on (spin, col)."""
+    _check(lib().qcd_hip_point_source(f.h, int(rhs), (ctypes.c_int * 4)(*site), int(spin), int(col)),
+           "qcd_hip_point_source")
+
+
 def spinor_gaussian(L, seed):
     V = int(np.prod(L))
     v = np.empty((V, 4, 3), dtype=np.complex128)
