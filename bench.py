@@ -66,9 +66,57 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
     return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
-            "config": {"traversal": "z-march" if (cfg[0] & 0xff) == 2 else ["xcd-contiguous", "xcd-y-slab"][cfg[0]],
+            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march, two lanes per site"][cfg[0] & 0xff],
                        "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
-                       "sched_fence": bool(cfg[1] & 16)}}
+                       "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}}
+
+
+def pmc_traffic(L, prec, cfg, timeout=150):
+    """HBM bytes per Dslash launch, from two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE) of qcd_amd.profile_run on the same lattice, precision and
+    launch configuration.  Each counter is scaled by what it reads for the plain
+    FP64 copy in the same run (192 B read + 192 B written per site).  Returns
+    (bytes or None, note)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    V = int(np.prod(L))
+    env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    got = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, "-m", "qcd_amd.profile_run", "--lattice", *map(str, L), "--prec", prec,
+                   "--map", str(cfg["map"]), "--occ", str(cfg["occ"]), "--iters", "5"]
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            hits = [os.path.join(a, f) for a, _, fs in os.walk(d) for f in fs if f.endswith("_counter_collection.csv")]
+            if r.returncode != 0 or not hits:
+                return None, f"rocprofv3 --pmc {counter} failed (rc {r.returncode})"
+            dsl, copy = [], []
+            with open(hits[0], newline="") as f:
+                for row in csv.DictReader(f):
+                    if row["Counter_Name"] != counter:
+                        continue
+                    if "qcdhip::k_dslash" in row["Kernel_Name"]:
+                        dsl.append(float(row["Counter_Value"]))
+                    elif "qcdhip::k_lincomb<0>" in row["Kernel_Name"]:
+                        copy.append(float(row["Counter_Value"]))
+            if not dsl or not copy or min(copy) <= 0:
+                return None, f"no Dslash/copy dispatches in the {counter} pass"
+            factor = 192.0 * V / (np.mean(copy) * 1024.0)
+            got[counter] = (np.mean(dsl) * 1024.0 * factor, factor)
+    fb, ff = got["FETCH_SIZE"]
+    wb, wf = got["WRITE_SIZE"]
+    return fb + wb, (f"FETCH_SIZE {fb / V:.0f} B/site (x{ff:.2f} copy calibration) + WRITE_SIZE {wb / V:.0f} "
+                     f"B/site (x{wf:.2f}); algorithmic {BYTES_PER_SITE[prec]} B/site")
 
 
 def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0.3, seed=2002):
@@ -169,6 +217,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure the headline kernel's HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one rank on one GPU; --gpus {args.gpus} is not supported "
@@ -193,6 +243,11 @@ def main():
     big = dslash_leg(Q, (96, 48, 48, 48), "f64", max(10, args.steps // 10), max(2, args.warmup // 10))
     dsl["f64_48c96"] = big
     log(f"f64 dslash 48^3x96: {big['ms']*1e3:.1f} us, {big['GBps']:.0f} GB/s")
+
+    traffic, traffic_note = None, "not measured (--no-pmc)"
+    if not args.no_pmc:
+        traffic, traffic_note = pmc_traffic(L, "f64", head["config"])
+        log(f"headline traffic: {traffic_note}")
 
     solves = None
     if not args.no_solve:
@@ -245,7 +300,8 @@ def main():
                                "theta=(pi,0,0,0), one RHS, inputs resident in HBM",
                    "lattice": "24^3x48", "sites": V, "bytes_per_site": 960},
         "roofline": {"bound": "hbm", "achieved": round(head["GBps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(head["GBps"] / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(head["GBps"] / HBM_PEAK_GBPS, 4),
+                     "traffic": None if traffic is None else round(traffic), "traffic_note": traffic_note},
         "cpu_baseline": cpu,
         "dslash": dsl,
         "solves": solves,

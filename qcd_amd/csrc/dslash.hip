@@ -47,7 +47,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const Cx<R> v = psi[3 * sp + c];
-          acc[sp][c] = Cx<R>{R(-2) * (mp4 * v.re - gg * v.im), R(-2) * (mp4 * v.im + gg * v.re)};
+          acc[sp][c] = diag_term(v, mp4, gg);
         }
       }
       const unsigned sb = neighbour<3, -1>(st, a), sf = neighbour<3, +1>(st, a);
@@ -136,7 +136,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 #pragma unroll
           for (int c = 0; c < 3; c++) {
             const Cx<R> v = psi[3 * sp + c];
-            acc[sp][c] = Cx<R>{R(-2) * (mp4 * v.re - gg * v.im), R(-2) * (mp4 * v.im + gg * v.re)};
+            acc[sp][c] = diag_term(v, mp4, gg);
           }
         }
         rebuild_add<3, +1>(acc, cz0, cz1);
@@ -170,6 +170,161 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
+// Bytes of dynamic LDS a MAP_ZTILE workgroup uses: (xb + 2) staged rows of T
+// sites, each site its 12 stored components and (FP16) its norm.
+inline size_t ztile_lds_bytes(int ps, unsigned T, unsigned xb) {
+  const size_t comp = ps == QCD_HIP_F64 ? 16 : ps == QCD_HIP_F32 ? 8 : 4;
+  return (size_t)(xb + 2) * T * (12 * comp + (ps == QCD_HIP_F16 ? 4 : 0));
+}
+
+// z-marching Dslash over tiles of whole t-rows.  Workgroup (tile g, y, z
+// chunk) owns rows x0 .. x0+nr-1 (nr <= xb) of plane y, lane (t, r) one site
+// of row x0 + r.  Each step it writes the stored spinors of its rows and of
+// the halo rows x0-1 and x0+nr (mod X) to LDS; the t hops (always within the
+// row) and the x hops then read LDS, and only the y and z hops and the links
+// use the vector memory path.  Same arithmetic and summation order as
+// k_dslash_zm: the results are bit-identical.
+template <int PS, int PG, int EPI, bool RED, int OCC>
+__global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
+k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
+            double *__restrict__ partials) {
+  using R = typename Prec<PS>::R;
+  using T = typename Prec<PS>::T;
+  extern __shared__ __align__(16) unsigned char qh_dyn_lds[];
+  const unsigned Lt = a.L[0], Lx = a.L[1], Ly = a.L[2];
+  const unsigned nslot = (a.xb + 2) * Lt;
+  T *stv = (T *)qh_dyn_lds;                      // [12][nslot]
+  float *stn = (float *)(stv + 12 * nslot);     // [nslot] (FP16 norms)
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const SpinorIn<PS> &in = rv.in, &b = rv.b;
+  const SpinorOut<PS> &out = rv.out;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned g = lb % a.nxg, r1 = lb / a.nxg;
+  const unsigned y = r1 % Ly, zc = r1 / Ly;
+  const unsigned x0 = g * a.xb;
+  const unsigned nr = min(a.xb, Lx - x0);
+  const unsigned tid = threadIdx.x;
+  const unsigned t = tid % Lt, xr = tid / Lt;
+  const bool active = xr < nr;
+  const unsigned P2 = Lt * Lx, P3 = P2 * Ly;
+  const unsigned z0 = zc * a.zlen;
+  const unsigned z1 = min(z0 + a.zlen, a.L[3]);
+  // halo rows: lanes h < 2 Lt (strided) load row x0-1 (h < Lt) or x0+nr (mod X)
+  const unsigned xlo = x0 == 0 ? Lx - 1 : x0 - 1;
+  const unsigned xhi = x0 + nr == Lx ? 0 : x0 + nr;
+  Site st;
+  st.valid = active;
+  st.t = t;
+  st.x = x0 + xr;
+  st.y = y;
+  const unsigned col = t + Lt * (st.x + Lx * y);
+  const bool tw = a.twisted[3] != 0;
+  const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
+  const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
+  const R mp4 = R(a.mp4), tm = R(a.tm);
+  // staged slot of (t', row j): j * Lt + t', rows j = 0 (x0-1), 1..nr, nr+1 (x0+nr)
+  const unsigned own = (xr + 1) * Lt + t;
+  const unsigned tf = (xr + 1) * Lt + (t + 1 == Lt ? 0 : t + 1);
+  const unsigned tb = (xr + 1) * Lt + (t == 0 ? Lt - 1 : t - 1);
+  const unsigned xf = (xr + 2) * Lt + t, xbk = xr * Lt + t;
+  double red = 0.0;
+  Cx<R> cz0[3], cz1[3];
+  if (active) {
+    const unsigned zm = z0 == 0 ? a.L[3] - 1 : z0 - 1;
+    const unsigned sm = col + P3 * zm;
+    Cx<R> psi[12], u[9];
+    in.load(sm, psi);
+    U.load(3, sm, u);
+    hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+  }
+  for (unsigned z = z0; z < z1; z++) {   // z0, z1 are uniform over the workgroup
+    st.z = z;
+    st.s = col + P3 * z;
+    const unsigned s = st.s;
+    T rc[12];
+    float nc = 0.f;
+    if (active) in.load_raw(s, rc, nc);
+    __syncthreads();   // every lane is done with the previous plane
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) stv[k * nslot + own] = rc[k];
+      if constexpr (PS == QCD_HIP_F16) stn[own] = nc;
+    }
+    for (unsigned h = tid; h < 2 * Lt; h += blockDim.x) {
+      const unsigned th = h < Lt ? h : h - Lt;
+      const unsigned xh = h < Lt ? xlo : xhi;
+      const unsigned slot = (h < Lt ? 0 : nr + 1) * Lt + th;
+      T rh[12];
+      float nh = 0.f;
+      in.load_raw(th + Lt * (xh + Lx * y) + P3 * z, rh, nh);
+#pragma unroll
+      for (int k = 0; k < 12; k++) stv[k * nslot + slot] = rh[k];
+      if constexpr (PS == QCD_HIP_F16) stn[slot] = nh;
+    }
+    __syncthreads();
+    if (active) {
+      Cx<R> acc[4][3];
+      {
+        Cx<R> psi[12], u[9], g0[3], g1[3];
+        SpinorIn<PS>::convert(rc, nc, psi);
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+          const R gg = sp < 2 ? tm : -tm;
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[sp][c] = diag_term(psi[3 * sp + c], mp4, gg);
+        }
+        rebuild_add<3, +1>(acc, cz0, cz1);
+        U.load(3, s, u);
+        // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
+        hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+        const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
+        in.load(sf, psi);
+        hop_half<3, -1, false>(g0, g1, psi, u, tw, php);
+        rebuild_add<3, -1>(acc, g0, g1);
+      }
+      constexpr bool F = (OCC & 16) != 0;
+      // t and x hops from the staged rows
+      {
+        auto staged = [&](unsigned slot, Cx<R> v[12]) {
+          T r[12];
+#pragma unroll
+          for (int k = 0; k < 12; k++) r[k] = stv[k * nslot + slot];
+          SpinorIn<PS>::convert(r, PS == QCD_HIP_F16 ? stn[slot] : 0.f, v);
+        };
+        Cx<R> psi[12], u[9];
+        if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+        staged(tf, psi);
+        U.load(0, s, u);
+        hop_accumulate<0, -1, false>(acc, psi, u, a.twisted[0] != 0, Cx<R>{(R)a.ph_re[0], (R)a.ph_im[0]});
+        staged(tb, psi);
+        U.load(0, neighbour<0, -1>(st, a), u);
+        hop_accumulate<0, +1, true>(acc, psi, u, a.twisted[0] != 0, Cx<R>{(R)a.ph_re[0], -(R)a.ph_im[0]});
+        if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+        staged(xf, psi);
+        U.load(1, s, u);
+        hop_accumulate<1, -1, false>(acc, psi, u, a.twisted[1] != 0, Cx<R>{(R)a.ph_re[1], (R)a.ph_im[1]});
+        staged(xbk, psi);
+        U.load(1, neighbour<1, -1>(st, a), u);
+        hop_accumulate<1, +1, true>(acc, psi, u, a.twisted[1] != 0, Cx<R>{(R)a.ph_re[1], -(R)a.ph_im[1]});
+      }
+      hop_dir<2, PS, PG, R, F>(acc, a, st, in, U);
+      if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+      Cx<R> o[12];
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[3 * sp + c] = Cx<R>{R(-0.5) * acc[sp][c].re, R(-0.5) * acc[sp][c].im};
+      apply_epilogue<EPI>(o, b, s);
+      if (RED) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+      }
+      out.store(s, o);
+    }
+  }
+  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
 template <int PS, int PG, int EPI, bool RED, int OCC>
 static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
@@ -178,6 +333,17 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
   if (a.map == MAP_ZMARCH)
     k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
                                                                   SpinorOut<PS>::of(out, ro), bi, partials);
+  else if (a.map == MAP_ZTILE) {
+    const size_t lds = ztile_lds_bytes(PS, a.L[0], a.xb);
+    static std::atomic<size_t> lds_set{0};   // per instantiation: raise the dynamic LDS limit once
+    if (lds > 65536 && lds > lds_set.load()) {
+      QH_CHECK(hipFuncSetAttribute((const void *)k_dslash_zt<PS, PG, EPI, RED, OCC>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      lds_set.store(lds);
+    }
+    k_dslash_zt<PS, PG, EPI, RED, OCC><<<gr, a.lanes, lds, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
+                                                                     SpinorOut<PS>::of(out, ro), bi, partials);
+  }
   else
     k_dslash<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
                                                                SpinorOut<PS>::of(out, ro), bi, partials);
@@ -238,7 +404,31 @@ DslashArgs make_args(qcd_hip_lattice lat, double m, double tm) {
 
 // Fill the traversal fields of a; returns the grid size.  For MAP_ZMARCH the
 // low byte of `map` is 2 and the z steps per lane are map >> 8.
-int set_traversal(DslashArgs &a, int map) {
+int set_traversal(DslashArgs &a, int map, int ps) {
+  if ((map & 0xff) == MAP_ZTILE) {
+    // rows per tile: the most whole t-rows in one workgroup, preferring a lane
+    // count that fills whole wavefronts; falls back to MAP_ZMARCH when no tile
+    // of whole rows fits a workgroup and the LDS
+    const unsigned Lt = a.L[0], Lx = a.L[1];
+    unsigned xb = std::min(Lx, (unsigned)kBlock / std::max(Lt, 1u));
+    for (unsigned r = xb; r >= 1; r--)
+      if ((Lt * r) % 64 == 0) {
+        xb = r;
+        break;
+      }
+    if (xb >= 1 && Lt <= (unsigned)kBlock && ztile_lds_bytes(ps, Lt, xb) <= 150 * 1024) {
+      const unsigned zlen = std::max(1u, std::min((unsigned)(map >> 8), a.L[3]));
+      a.map = MAP_ZTILE;
+      a.zlen = zlen;
+      a.xb = xb;
+      a.nxg = (Lx + xb - 1) / xb;
+      a.lanes = (Lt * xb + 63) / 64 * 64;
+      a.col_blocks = a.nxg * a.L[2];
+      const unsigned nz = (a.L[3] + zlen - 1) / zlen;
+      return (int)(a.col_blocks * nz);
+    }
+    map = MAP_ZMARCH | (map & ~0xff);
+  }
   if ((map & 0xff) == MAP_ZMARCH) {
     const unsigned zlen = std::max(1u, std::min((unsigned)(map >> 8), a.L[3]));
     const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
@@ -309,7 +499,7 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
                       qcd_hip_field g, qcd_hip_field b, int rb, double m, double tm, double *partials,
                       hipStream_t stream, int *grid_out, const Batch &bt = kOne) {
   DslashArgs a = make_args(in->lat, m, tm);
-  const int grid = set_traversal(a, map);
+  const int grid = set_traversal(a, map, in->prec);
   a.ids = bt.ids;
   a.count = (unsigned)bt.count;
   a.in_rs = (unsigned)bt.in_rs;
@@ -331,12 +521,16 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
 
 // untuned default: z-march of 6 steps per lane
 static const int kDefaultMap = MAP_ZMARCH | (6 << 8);
-// Candidates: z-march lengths (1 = one site per lane).  The per-site
-// traversals MAP_XCD / MAP_SLAB stay selectable but are never faster: their
-// straight-line hop sequence lets the compiler hoist every load and spill.
+// Candidates: z-march lengths (1 = one site per lane), without and with the
+// LDS-staged (t, x) neighbourhood.  The per-site traversals MAP_XCD / MAP_SLAB stay selectable but are
+// never faster: their straight-line hop sequence lets the compiler hoist every
+// load and spill.
 static const int kMaps[] = {MAP_ZMARCH | (1 << 8), MAP_ZMARCH | (2 << 8), MAP_ZMARCH | (3 << 8),
-                           MAP_ZMARCH | (4 << 8), MAP_ZMARCH | (6 << 8), MAP_ZMARCH | (8 << 8),
-                           MAP_ZMARCH | (12 << 8), MAP_ZMARCH | (16 << 8)};
+                            MAP_ZMARCH | (4 << 8), MAP_ZMARCH | (6 << 8), MAP_ZMARCH | (8 << 8),
+                            MAP_ZMARCH | (12 << 8), MAP_ZMARCH | (16 << 8),
+                            MAP_ZTILE | (1 << 8),  MAP_ZTILE | (2 << 8),  MAP_ZTILE | (3 << 8),
+                            MAP_ZTILE | (4 << 8),  MAP_ZTILE | (6 << 8),  MAP_ZTILE | (8 << 8),
+                            MAP_ZTILE | (12 << 8), MAP_ZTILE | (16 << 8)};
 static const int kOccs[] = {0, 2, 3, 16};
 
 int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force) {
@@ -446,10 +640,11 @@ int dslash_prepare(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g) {
 // an upper bound of the Dslash grid over every traversal candidate
 int dslash_grid_bound(qcd_hip_lattice lat) {
   int most = 0;
-  for (int map : {(int)MAP_XCD, (int)MAP_SLAB, MAP_ZMARCH | (1 << 8)}) {   // z-march of 1 has the most groups
-    DslashArgs a = make_args(lat, 0.0, 0.0);
-    most = std::max(most, set_traversal(a, map));
-  }
+  for (int map : {(int)MAP_XCD, (int)MAP_SLAB, MAP_ZMARCH | (1 << 8), MAP_ZTILE | (1 << 8)})   // z-march of 1 has the most groups
+    for (int ps : {QCD_HIP_F64, QCD_HIP_F32, QCD_HIP_F16}) {
+      DslashArgs a = make_args(lat, 0.0, 0.0);
+      most = std::max(most, set_traversal(a, map, ps));
+    }
   return most;
 }
 
@@ -457,7 +652,7 @@ int dslash_grid(qcd_hip_field in, qcd_hip_field g) {
   TuneVal t;
   const bool have = tune_get(key_of(in, g), &t);
   DslashArgs a = make_args(in->lat, 0.0, 0.0);
-  return set_traversal(a, have ? t.map : kDefaultMap);
+  return set_traversal(a, have ? t.map : kDefaultMap, in->prec);
 }
 
 }  // namespace qcdhip
@@ -528,12 +723,14 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
   return 0;
 }
 
-// Force a configuration (map = 0 contiguous-per-XCD, 1 = y-slab per XCD;
-// occ = 0 default, 3 or 4 minimum waves per SIMD) for these fields' key.
+// Force a configuration for these fields' key: map = 0 contiguous per XCD,
+// 1 y-slab per XCD, 2 | (n << 8) z-march of n steps, 3 | (n << 8) z-march of
+// n steps with LDS-staged t and x neighbours; occ = 0 compiler default, 2 or 3 minimum
+// waves per SIMD, 16 scheduling fence between hop directions.
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ) {
   QH_TRY(ensure_init());
   const int m0 = map & 0xff;
-  if (!in || !gauge || m0 < 0 || m0 > 2 || (m0 == 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
+  if (!in || !gauge || m0 < 0 || m0 > 3 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
     set_error("dslash_set_config: bad arguments");
     return 1;
   }

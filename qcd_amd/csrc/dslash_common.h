@@ -95,18 +95,29 @@ struct SpinorIn {
   static __device__ __forceinline__ unsigned offset(unsigned s) {
     return ((s / kTile) * (12u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
   }
-  __device__ __forceinline__ void load(unsigned s, Cx<R> v[12]) const {
+  // stored values of site s (and its FP16 norm)
+  __device__ __forceinline__ void load_raw(unsigned s, T r[12], float &n) const {
     const unsigned off = offset(s);
 #pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(p[k], off));
+    for (int k = 0; k < 12; k++) r[k] = ld_plane(p[k], off);
+    if constexpr (P == QCD_HIP_F16) n = ld_plane(norm, s * 4u);
+  }
+  static __device__ __forceinline__ void convert(const T r[12], float n, Cx<R> v[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(r[k]);
     if constexpr (P == QCD_HIP_F16) {
-      const R n = ld_plane(norm, s * 4u);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
-        v[k].re *= n;
-        v[k].im *= n;
+        v[k].re *= (R)n;
+        v[k].im *= (R)n;
       }
     }
+  }
+  __device__ __forceinline__ void load(unsigned s, Cx<R> v[12]) const {
+    T r[12];
+    float n = 0.f;
+    load_raw(s, r, n);
+    convert(r, n, v);
   }
 };
 
@@ -184,7 +195,11 @@ struct GaugeIn {
 //  MAP_ZMARCH: each lane owns one (t, x, y) column and steps through zlen
 //              consecutive z values, carrying the centre spinor and the
 //              backward z hop in registers (2.5D blocking).
-enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2 };
+//  MAP_ZTILE : z-march of a tile of whole t-rows: a workgroup owns xb
+//              consecutive x-rows (all t) of one y and stages them, with one
+//              halo row on each side, in LDS every step, so t and x
+//              neighbours never go through the vector memory path.
+enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3 };
 
 struct DslashArgs {
   unsigned V;
@@ -197,7 +212,8 @@ struct DslashArgs {
   unsigned slab_y0[kNumXcd + 1];   // MAP_SLAB: y range of each XCD
   unsigned slab_blocks;            // MAP_SLAB: workgroups per XCD
   unsigned zlen;                   // MAP_ZMARCH: z steps per lane
-  unsigned col_blocks;             // MAP_ZMARCH: workgroups per z chunk
+  unsigned col_blocks;             // MAP_ZMARCH, MAP_ZTILE: workgroups per z chunk
+  unsigned xb, nxg, lanes;         // MAP_ZTILE: x-rows per tile, tiles per x line, lanes per workgroup
   // right-hand-side batch: grid row y works on slot = ids ? ids[y] : y, i.e.
   // on vectors in + slot*in_rs, out + slot*out_rs, b + slot*b_rs
   const int *ids;
@@ -316,20 +332,42 @@ __device__ __forceinline__ Cx<R> ipow(Cx<R> v) {
 template <typename R>
 __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return Cx<R>{a.re + b.re, a.im + b.im}; }
 
-// y = U x (DAG = false) or U^+ x (DAG = true) for one colour vector
+// y = U x (DAG = false) or U^+ x (DAG = true) for one colour vector.  Every
+// complex multiply-add is written as explicit fused multiply-adds so that all
+// kernels (whatever the surrounding code) round identically.
 template <bool DAG, typename R>
 __device__ __forceinline__ void su3_mul(Cx<R> y[3], const Cx<R> u[9], const Cx<R> x[3]) {
 #pragma unroll
   for (int r = 0; r < 3; r++) {
-    R re = R(0), im = R(0);
+    R re, im;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const Cx<R> m = DAG ? Cx<R>{u[3 * k + r].re, -u[3 * k + r].im} : u[3 * r + k];
-      re += m.re * x[k].re - m.im * x[k].im;
-      im += m.re * x[k].im + m.im * x[k].re;
+      if (k == 0) {
+        re = m.re * x[k].re;
+        im = m.re * x[k].im;
+      } else {
+        re = fma(m.re, x[k].re, re);
+        im = fma(m.re, x[k].im, im);
+      }
+      re = fma(-m.im, x[k].im, re);
+      im = fma(m.im, x[k].re, im);
     }
     y[r] = Cx<R>{re, im};
   }
+}
+
+// v * ph
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul_ph(Cx<R> v, Cx<R> ph) {
+  return Cx<R>{fma(v.re, ph.re, -(v.im * ph.im)), fma(v.re, ph.im, v.im * ph.re)};
+}
+
+// -2 (m + 4 + i g tm) v: the diagonal term of one spin component (g = +1 for
+// spins 0, 1 and -1 for spins 2, 3)
+template <typename R>
+__device__ __forceinline__ Cx<R> diag_term(Cx<R> v, R mp4, R gtm) {
+  return Cx<R>{R(-2) * fma(mp4, v.re, -(gtm * v.im)), R(-2) * fma(mp4, v.im, gtm * v.re)};
 }
 
 // accumulate one hop: phase * link * (1 + S g_MU) psi, rebuilt to 4 spins
@@ -351,8 +389,8 @@ __device__ __forceinline__ void hop_accumulate(Cx<R> acc[4][3], const Cx<R> psi[
   // the live registers.
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
-    g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
+    g0[c] = cmul_ph(g0[c], ph);
+    g1[c] = cmul_ph(g1[c], ph);
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -382,8 +420,8 @@ __device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> p
   // the live registers.
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    g0[c] = Cx<R>{g0[c].re * ph.re - g0[c].im * ph.im, g0[c].re * ph.im + g0[c].im * ph.re};
-    g1[c] = Cx<R>{g1[c].re * ph.re - g1[c].im * ph.im, g1[c].re * ph.im + g1[c].im * ph.re};
+    g0[c] = cmul_ph(g0[c], ph);
+    g1[c] = cmul_ph(g1[c], ph);
   }
 }
 
@@ -458,8 +496,8 @@ __device__ __forceinline__ void block_reduce_store(double v, double *partials, u
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; w++) t += wsum[w];
+    const int nw = (blockDim.x + 63) / 64;
+    for (int w = 0; w < nw; w++) t += wsum[w];
     partials[slot] = t;
   }
 }

@@ -227,34 +227,48 @@ __global__ void __launch_bounds__(1024) k_scal_dots(GcrScal *scs, const double *
 
 // after the multi-axpy of iteration k: normalisation, c_k and the
 // coefficient recurrence of invert.c:83-94
+// Row k is scaled by s_k first; then every a[k][j] is updated from the scaled
+// row and the finished rows l < k, so the j are independent: thread j does
+// one of them, with rows 0..k-1 staged in LDS (packed lower triangle).
 __global__ void __launch_bounds__(1024) k_scal_norm(GcrScal *scs, const double *part, size_t rowsz, unsigned nb, int k,
                                                     int n, const int *ids) {
   __shared__ double sums[3];
+  __shared__ double2 rowk[kN];
+  __shared__ double diag[kN];
+  __shared__ double2 tri[kN * (kN - 1) / 2];   // a[l][j], j < l < k, at l (l - 1) / 2 + j
   GcrScal *sc = scs + ids[blockIdx.x];
   wave_sums(part + blockIdx.x * rowsz, 3, nb, sums);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double sk = 1.0 / sqrt(sums[0]);
-  if (!(sk == sk) || isinf(sk)) sc->bad = 1;   // Krylov space is sick (invert.c:75-79)
-  sc->s[k] = sk;
-  sc->a[k * n + k] = make_double2(sk, 0.0);
-  const double2 ck = make_double2(sk * sums[1], sk * sums[2]);
-  sc->c[k] = ck;
-  for (int j = 0; j < k; j++) {
-    double2 &a = sc->a[k * n + j];
-    a = make_double2(a.x * sk, a.y * sk);
+  const int tid = threadIdx.x;
+  for (int base = 0; base < k; base += 1024 / 64) {
+    const int l = base + (tid >> 6), j = tid & 63;
+    if (l < k && j < l) tri[l * (l - 1) / 2 + j] = sc->a[l * n + j];
   }
-  for (int j = 0; j < k; j++) {
-    double2 t = sc->a[k * n + j];
-    const double ajj = sc->a[j * n + j].x;
-    t = make_double2(t.x * ajj, t.y * ajj);
+  if (tid < k) diag[tid] = sc->a[tid * n + tid].x;
+  __syncthreads();
+  const double sk = 1.0 / sqrt(sums[0]);
+  if (tid < k) {
+    const double2 a = sc->a[k * n + tid];
+    rowk[tid] = make_double2(a.x * sk, a.y * sk);
+  }
+  __syncthreads();
+  if (tid < k) {
+    const int j = tid;
+    double2 t = rowk[j];
+    t = make_double2(t.x * diag[j], t.y * diag[j]);
     for (int l = j + 1; l < k; l++) {
-      const double2 p = cmul2(sc->a[k * n + l], sc->a[l * n + j]);
+      const double2 p = cmul2(rowk[l], tri[l * (l - 1) / 2 + j]);
       t = make_double2(t.x + p.x, t.y + p.y);
     }
     sc->a[k * n + j] = t;
   }
-  sc->alpha = make_double2(-ck.x * sk, -ck.y * sk);
+  if (tid == 0) {
+    if (!(sk == sk) || isinf(sk)) sc->bad = 1;   // Krylov space is sick (invert.c:75-79)
+    sc->s[k] = sk;
+    sc->a[k * n + k] = make_double2(sk, 0.0);
+    const double2 ck = make_double2(sk * sums[1], sk * sums[2]);
+    sc->c[k] = ck;
+    sc->alpha = make_double2(-ck.x * sk, -ck.y * sk);
+  }
 }
 
 // beta_j = sum_{k >= j} c_k a[k][j]; keep_q1 drops the k = 0 term as
