@@ -28,7 +28,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
   using R = typename Prec<PS>::R;
   const Site st = site_of_lane(a);
   const unsigned s = st.s;
-  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, blockIdx.y);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
   const SpinorOut<PS> &out = rv.out;
   double red = 0.0;
@@ -92,10 +92,11 @@ __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
             double *__restrict__ partials) {
   using R = typename Prec<PS>::R;
-  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const BlockRow br = block_row(a);
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, br.row);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
   const SpinorOut<PS> &out = rv.out;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned lb = br.blk;
   const unsigned cb = lb % a.col_blocks, zc = lb / a.col_blocks;
   const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
   const unsigned col = cb * kBlock + threadIdx.x;
@@ -167,7 +168,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
       out.store(s, o);
     }
   }
-  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
+  if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
 
 // Bytes of dynamic LDS a MAP_ZTILE workgroup uses: (xb + 2) staged rows of T
@@ -195,10 +196,11 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   const unsigned nslot = (a.xb + 2) * Lt;
   T *stv = (T *)qh_dyn_lds;                      // [12][nslot]
   float *stn = (float *)(stv + 12 * nslot);     // [nslot] (FP16 norms)
-  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all);
+  const BlockRow br = block_row(a);
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, br.row);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
   const SpinorOut<PS> &out = rv.out;
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned lb = br.blk;
   const unsigned g = lb % a.nxg, r1 = lb / a.nxg;
   const unsigned y = r1 % Ly, zc = r1 / Ly;
   const unsigned x0 = g * a.xb;
@@ -322,14 +324,16 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
       out.store(s, o);
     }
   }
-  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
+  if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
 
 template <int PS, int PG, int EPI, bool RED, int OCC>
 static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
-  const dim3 gr((unsigned)grid, (unsigned)a.count);
+  // z-march traversals interleave the batch rows into grid.x (see block_row)
+  const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE;
+  const dim3 gr(zmarch ? (unsigned)grid * a.count : (unsigned)grid, zmarch ? 1u : (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
     k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
                                                                   SpinorOut<PS>::of(out, ro), bi, partials);
@@ -505,6 +509,7 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
   a.in_rs = (unsigned)bt.in_rs;
   a.out_rs = (unsigned)bt.out_rs;
   a.b_rs = (unsigned)bt.b_rs;
+  a.nblk = (unsigned)grid;
   if (grid_out) *grid_out = grid;
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)

@@ -218,7 +218,8 @@ struct DslashArgs {
   // on vectors in + slot*in_rs, out + slot*out_rs, b + slot*b_rs
   const int *ids;
   unsigned in_rs, out_rs, b_rs;
-  unsigned count;                  // grid rows (host side)
+  unsigned count;                  // batch rows (z-march: interleaved into grid.x, else grid.y)
+  unsigned nblk;                   // workgroups per batch row
 };
 
 template <int PS>
@@ -229,13 +230,25 @@ struct RhsView {
 
 template <int PS>
 __device__ __forceinline__ RhsView<PS> rhs_view(const DslashArgs &a, const SpinorIn<PS> &in, const SpinorOut<PS> &out,
-                                                const SpinorIn<PS> &b) {
-  const unsigned slot = a.ids ? (unsigned)a.ids[blockIdx.y] : blockIdx.y;
+                                                const SpinorIn<PS> &b, unsigned row) {
+  const unsigned slot = a.ids ? (unsigned)a.ids[row] : row;
   RhsView<PS> v;
   v.in = in.advanced(slot * a.in_rs);
   v.out = out.advanced(slot * a.out_rs);
   v.b = b.advanced(slot * a.b_rs);
   return v;
+}
+
+// Block decomposition of the z-march traversals: one 1D grid of nblk * count
+// workgroups, remapped per XCD, with the right-hand side the fastest index,
+// so the count right-hand sides of one tile run together on one XCD and share
+// its links through that XCD's L2.  row = batch row, blk = tile index.
+struct BlockRow {
+  unsigned row, blk;
+};
+__device__ __forceinline__ BlockRow block_row(const DslashArgs &a) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  return BlockRow{lb % a.count, lb / a.count};
 }
 
 struct Site {
