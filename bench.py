@@ -120,7 +120,7 @@ def pmc_traffic(L, prec, cfg, timeout=150):
 
 
 def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0.3, seed=2002,
-               sites=((0, 0, 0, 0),)):
+               sites=((0, 0, 0, 0),), eo=0):
     """Propagators of 12 point sources (spin x colour) at each site, every rhs
     solved to ||b - D x|| / ||b|| <= 1e-10 and checked in FP64 afterwards."""
     lat = Q.Lattice(L)
@@ -132,10 +132,10 @@ def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0
         for r in range(12):
             Q.point_source_device(b, 12 * i + r, site, r // 3, r % 3)
     x = Q.Spinor(lat, nrhs=nr)
-    S = Q.Solver(lat, g, n_restart=n_restart, tol=1e-10, inner_prec=prec, batch=batch, max_outer=20000)
+    S = Q.Solver(lat, g, n_restart=n_restart, tol=1e-10, inner_prec=prec, batch=batch, max_outer=20000, eo=eo)
     b1 = Q.Spinor(lat).upload(Q.point_source(L, (0, 0, 0, 0), 0, 0))
     x1 = Q.Spinor(lat)
-    S1 = Q.Solver(lat, g, n_restart=n_restart, tol=1e-4, inner_prec=prec, batch=1)
+    S1 = Q.Solver(lat, g, n_restart=n_restart, tol=1e-4, inner_prec=prec, batch=1, eo=eo)
     S1.solve(x1, b1, kappa)   # tunes the operators outside the timed region
     Q.synchronize()
     t = time.perf_counter()
@@ -145,7 +145,8 @@ def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0
     worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(nr))
     assert worst <= 1e-10, f"{prec} solve not converged: {worst}"
     return {"solves_per_s": nr / el, "seconds": el, "rhs": nr, "batch": batch, "n_restart": n_restart,
-            "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
+            "even_odd": bool(eo), "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"],
+            "worst_b_minus_Dx": worst}
 
 
 def config3_sites(L=(48, 24, 24, 24), seed=3003):
@@ -225,8 +226,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-solve", action="store_true", help="skip the solver legs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
-    ap.add_argument("--serve-requests", type=int, default=12)
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the headline kernel's HBM traffic")
     args = ap.parse_args()
@@ -262,27 +263,35 @@ def main():
     solves = None
     if not args.no_solve:
         solves = {"config": "16^3x32, kappa 0.130, eps 0.3, 12 point sources at the origin, tol 1e-10 (FP64)"}
-        solves["fp64_one_at_a_time"] = solves_leg(Q, "f64", 1, 32)
-        solves["mixed_f32_one_at_a_time"] = solves_leg(Q, "f32", 1, 32)
-        solves["mixed_f16_batched12"] = solves_leg(Q, "f16", 12, 32)
-        for k in ("fp64_one_at_a_time", "mixed_f32_one_at_a_time", "mixed_f16_batched12"):
-            log(f"solves {k}: {solves[k]['solves_per_s']:.2f}/s ({solves[k]['gcr_iterations']} GCR iterations)")
-        c3 = config3_sites()
-        solves["config3"] = {"config": "24^3x48, kappa 0.132, eps 0.3, 48 rhs = 12 point sources at each of 4 "
-                                       f"seeded sites {c3}, tol 1e-10 (FP64)",
-                             "mixed_f16_batched48": solves_leg(Q, "f16", 48, 32, L=(48, 24, 24, 24), kappa=0.132,
-                                                               eps=0.3, seed=3003, sites=c3)}
-        log(f"solves config3 mixed_f16_batched48: {solves['config3']['mixed_f16_batched48']['solves_per_s']:.2f}/s")
+        legs = [("fp64_one_at_a_time", "f64", 1, 0), ("mixed_f32_one_at_a_time", "f32", 1, 0),
+                ("mixed_f16_batched12", "f16", 12, 0), ("fp64_eo_one_at_a_time", "f64", 1, 1),
+                ("mixed_f32_eo_one_at_a_time", "f32", 1, 1), ("mixed_f16_eo_batched12", "f16", 12, 1)]
+        for name, prec, batch, eo in legs:
+            solves[name] = solves_leg(Q, prec, batch, 32, eo=eo)
+            log(f"solves {name}: {solves[name]['solves_per_s']:.2f}/s ({solves[name]['gcr_iterations']} GCR iterations)")
         base = solves["fp64_one_at_a_time"]["solves_per_s"]
+        base_eo = solves["fp64_eo_one_at_a_time"]["solves_per_s"]
         solves["speedup_mixed_f32_vs_fp64"] = solves["mixed_f32_one_at_a_time"]["solves_per_s"] / base
         solves["speedup_mixed_f16_batched_vs_fp64"] = solves["mixed_f16_batched12"]["solves_per_s"] / base
+        solves["speedup_mixed_f16_eo_batched_vs_fp64_eo"] = solves["mixed_f16_eo_batched12"]["solves_per_s"] / base_eo
+        solves["speedup_mixed_f16_eo_batched_vs_fp64"] = solves["mixed_f16_eo_batched12"]["solves_per_s"] / base
+        c3 = config3_sites()
+        L3 = (48, 24, 24, 24)
+        solves["config3"] = {"config": "24^3x48, kappa 0.132, eps 0.3, 48 rhs = 12 point sources at each of 4 "
+                                       f"seeded sites {c3}, tol 1e-10 (FP64)"}
+        for name, prec, batch in (("fp64_eo_one_at_a_time", "f64", 1), ("mixed_f16_eo_batched48", "f16", 48)):
+            solves["config3"][name] = solves_leg(Q, prec, batch, 32, L=L3, kappa=0.132, eps=0.3, seed=3003, sites=c3,
+                                                 eo=1)
+            log(f"solves config3 {name}: {solves['config3'][name]['solves_per_s']:.2f}/s")
+        solves["config3"]["speedup_mixed_vs_fp64"] = (solves["config3"]["mixed_f16_eo_batched48"]["solves_per_s"] /
+                                                      solves["config3"]["fp64_eo_one_at_a_time"]["solves_per_s"])
 
     serve = None
     if not args.no_serve:
         reqs = serve_stream(args.serve_requests, args.serve_rate, 777)
         serve = {"stream": f"{args.serve_requests} requests, Poisson rate {args.serve_rate}/s, volumes "
                            "{8^3x16,16^3x32,24^3x48}, kappa U[0.120,0.132], eps U[0.2,0.35], rhs {1,12}, "
-                           "4 workers, 200 GB budget, tol 1e-10"}
+                           "4 workers, 200 GB budget, tol 1e-10, even-odd preconditioned inner solves"}
         for prec in ("f16", "f64"):
             serve["mixed_f16" if prec == "f16" else "fp64"] = serve_leg(Q, reqs, prec)
             s = serve["mixed_f16" if prec == "f16" else "fp64"]

@@ -133,6 +133,10 @@ typedef struct {
                           solution exactly as invert.c:50-53 does */
    int    batch;       /* right-hand sides solved together (1 = one at a
                           time as invert.c); workspace grows with it */
+   int    eo;          /* 1: even-odd preconditioning: the inner GCR solves
+                          the Schur complement D^ = M - H_oe M^-1 H_eo on odd
+                          sites (half-length vectors) and the outer loop the
+                          full D x = b; needs every extent even; op is then D */
 } qcd_hip_solver_params;
 typedef struct {
    int    outer;        /* outer cycles (summed over rhs) */
@@ -156,6 +160,13 @@ int  qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field gauge64, dou
 /* ||D x[rx] - b[rb]|| / ||b[rb]|| in FP64, m = 1/(2 kappa) - 4 (b_minus_Dx.c:152-167) */
 int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
                         double kappa, double mu, double *rel);
+
+/* Even-odd Schur complement D^ = M - H_oe M^-1 H_eo (M = m + 4 + i tm g5, H
+ * the hopping term of D) applied to every rhs of in: in and out hold the odd
+ * sites of a lattice {T, X, Y, Z} as fields of its half lattice
+ * {T/2, X, Y, Z} (odd site l at entry l >> 1); gauge is the full-lattice
+ * field.  No reference counterpart (the reference has no even-odd form). */
+int qcd_hip_dslash_schur(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, double m, double tm);
 
 /* ------------------------------------------------------------ serving
  * Continuous serving of propagator-solve requests (no reference

@@ -67,7 +67,7 @@ class SolverParams(ctypes.Structure):
     """qcd_hip_solver_params (include/qcd_hip.h)."""
     _fields_ = [("n_restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("tol", ctypes.c_double),
                 ("inner_prec", ctypes.c_int), ("op", ctypes.c_int), ("keep_q1", ctypes.c_int),
-                ("batch", ctypes.c_int)]
+                ("batch", ctypes.c_int), ("eo", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -121,6 +121,7 @@ def lib():
         "qcd_hip_gauge_generate_host": (i, [_dp, lg, d, ctypes.c_uint64]),
         "qcd_hip_spinor_gaussian_host": (i, [_dp, lg, ctypes.c_uint64]),
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
+        "qcd_hip_dslash_schur": (i, [_vp, _vp, _vp, d, d]),
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
         "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
         "qcd_hip_dslash_set_config": (i, [_vp, _vp, i, i]),
@@ -332,6 +333,36 @@ def point_source(L, site, spin, col):
 # ------------------------------------------------------------ operators / BLAS
 def dslash(out, inp, gauge, m, tm=0.0, op=OP_D):
     _check(lib().qcd_hip_dslash(out.h, inp.h, gauge.h, float(m), float(tm), int(op)), "qcd_hip_dslash")
+
+
+def dslash_schur(out, inp, gauge, m, tm=0.0):
+    """Even-odd Schur complement D^ = M - H_oe M^-1 H_eo on odd-site half-lattice fields."""
+    _check(lib().qcd_hip_dslash_schur(out.h, inp.h, gauge.h, float(m), float(tm)), "qcd_hip_dslash_schur")
+
+
+def eo_split(a, L):
+    """(even, odd) half-lattice arrays of a full-lattice array [V, ...]: entry l >> 1 of parity p."""
+    T, X, Y, Z = L
+    t, x, y, z = np.meshgrid(np.arange(T), np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+    lex = (t + T * (x + X * (y + Y * z))).reshape(-1)
+    par = ((t + x + y + z) % 2).reshape(-1)
+    order = np.argsort(lex)
+    lex, par = lex[order], par[order]
+    return a[lex[par == 0]], a[lex[par == 1]]
+
+
+def eo_merge(ev, od, L):
+    """Inverse of eo_split."""
+    T, X, Y, Z = L
+    V = T * X * Y * Z
+    l = np.arange(V)
+    t, x = l % T, (l // T) % X
+    y, z = (l // (T * X)) % Y, l // (T * X * Y)
+    par = (t + x + y + z) % 2
+    out = np.empty((V,) + ev.shape[1:], dtype=ev.dtype)
+    out[par == 0] = ev
+    out[par == 1] = od
+    return out
 
 
 def dslash_time(out, inp, gauge, m, tm=0.0, op=OP_D, warmup=5, iters=20):

@@ -72,6 +72,9 @@ int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_fi
 int gauss_iterate(qcd_hip_field v, qcd_hip_field g, qcd_hip_field tmp, double alpha, int t0, int niter,
                   hipStream_t st);
 
+// even-odd preconditioning applies (every extent even), eo.hip
+bool eo_supported(qcd_hip_lattice lat);
+
 // device-side synthetic inputs (asynchronous on st)
 int gauge_generate_device(qcd_hip_field g, double eps, uint64_t seed, hipStream_t st);
 int point_source_device(qcd_hip_field f, int rhs, long site, int spin, int col, hipStream_t st);

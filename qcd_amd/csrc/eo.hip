@@ -1,0 +1,354 @@
+// eo.hip -- even-odd (Schur complement) form of the twisted-mass Wilson
+// operator, used by the inner solve of the propagator solver.
+//
+// With sites split by parity p = (t + x + y + z) mod 2 and D = M + H, where
+// M = (m + 4) + i tm g5 is site-diagonal and the hopping term H only couples
+// opposite parities,
+//   D = [[M, H_eo], [H_oe, M]],   D^ = M - H_oe M^-1 H_eo   (on odd sites).
+// D x = b is solved as D^ x_o = b_o - H_oe M^-1 b_e, x_e = M^-1 (b_e - H_eo x_o).
+// Half-lattice fields hold the sites of one parity: site l of parity p is
+// entry l >> 1 (every extent even, so the t-rows alternate parities).
+// The hopping sum uses the projector/link/phase arithmetic of the full Dslash.
+#include "blas_common.h"
+
+namespace qcdhip {
+
+// EO epilogues: out = M^-1 H in | M b - H in | b - H in | M^-1 (b - H in)
+enum { EO_MINV = 0, EO_SCHUR = 1, EO_SUB = 2, EO_SUB_MINV = 3 };
+
+struct EoArgs {
+  DslashArgs d;            // full-lattice geometry, phases, mass, batch
+  unsigned Vh;             // sites per parity
+  unsigned par;            // parity of the output sites
+  double mi_re, mi_im;     // 1 / (m + 4 + i tm)
+};
+
+// the site of parity par stored at half-lattice entry h
+__device__ __forceinline__ Site eo_site(unsigned h, unsigned par, const DslashArgs &a) {
+  Site st;
+  const unsigned l0 = 2 * h;
+  const unsigned r = l0 / a.L[0];
+  const unsigned t0 = l0 - r * a.L[0];
+  const unsigned r2 = r / a.L[1];
+  st.x = r - r2 * a.L[1];
+  st.z = r2 / a.L[2];
+  st.y = r2 - st.z * a.L[2];
+  const unsigned odd = (par + st.x + st.y + st.z) & 1u;
+  st.t = t0 + odd;
+  st.s = l0 + odd;
+  st.valid = true;
+  return st;
+}
+
+// M v (inv = false) or M^-1 v (inv = true) on one site: spins 0, 1 carry
+// m + 4 + i tm, spins 2, 3 its conjugate (g5 = diag(1, 1, -1, -1))
+template <bool INV, typename R>
+__device__ __forceinline__ void eo_mass(Cx<R> v[12], const EoArgs &e) {
+  const R are = INV ? (R)e.mi_re : (R)e.d.mp4;
+  const R aim = INV ? (R)e.mi_im : (R)e.d.tm;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const R im = k < 6 ? aim : -aim;
+    v[k] = Cx<R>{fma(are, v[k].re, -(im * v[k].im)), fma(are, v[k].im, im * v[k].re)};
+  }
+}
+
+// both hops of direction MU into acc, the neighbour spinors read from the
+// half-lattice field of the other parity
+template <int MU, int PS, int PG, typename R>
+__device__ __forceinline__ void eo_hop_dir(Cx<R> acc[4][3], const DslashArgs &a, const Site &st,
+                                           const SpinorIn<PS> &in, const GaugeIn<PG> &U) {
+  const bool tw = a.twisted[MU] != 0;
+  const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
+  const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
+  Cx<R> psi[12], u[9];
+  {
+    const unsigned sf = neighbour<MU, +1>(st, a);
+    in.load(sf >> 1, psi);
+    U.load(MU, st.s, u);
+    hop_accumulate<MU, -1, false>(acc, psi, u, tw, php);
+  }
+  {
+    const unsigned sb = neighbour<MU, -1>(st, a);
+    in.load(sb >> 1, psi);
+    U.load(MU, sb, u);
+    hop_accumulate<MU, +1, true>(acc, psi, u, tw, phm);
+  }
+}
+
+// one lane per output site; grid.y runs over the batch rows (DslashArgs::ids)
+template <int PS, int PG, int EPI, bool RED>
+__global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all, GaugeIn<PG> U,
+                                                   SpinorOut<PS> out_all, SpinorIn<PS> b_all,
+                                                   double *__restrict__ partials) {
+  using R = typename Prec<PS>::R;
+  const DslashArgs &a = e.d;
+  const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, blockIdx.y);
+  const unsigned h = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double red = 0.0;
+  if (h < e.Vh) {
+    const Site st = eo_site(h, e.par, a);
+    Cx<R> acc[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[q][c] = Cx<R>{R(0), R(0)};
+    eo_hop_dir<0, PS, PG, R>(acc, a, st, rv.in, U);
+    eo_hop_dir<1, PS, PG, R>(acc, a, st, rv.in, U);
+    eo_hop_dir<2, PS, PG, R>(acc, a, st, rv.in, U);
+    eo_hop_dir<3, PS, PG, R>(acc, a, st, rv.in, U);
+    // H in = -1/2 acc
+    Cx<R> o[12];
+    if constexpr (EPI == EO_MINV) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[3 * q + c] = Cx<R>{R(-0.5) * acc[q][c].re, R(-0.5) * acc[q][c].im};
+      eo_mass<true>(o, e);
+    } else {
+      Cx<R> bv[12];
+      rv.b.load(h, bv);
+      if constexpr (EPI == EO_SCHUR) eo_mass<false>(bv, e);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          o[3 * q + c] = Cx<R>{fma(R(0.5), acc[q][c].re, bv[3 * q + c].re), fma(R(0.5), acc[q][c].im, bv[3 * q + c].im)};
+      if constexpr (EPI == EO_SUB_MINV) eo_mass<true>(o, e);
+    }
+    if (RED) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+    }
+    rv.out.store(h, o);
+  }
+  if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// full FP64 vector f0 + ids[row] -> half vectors ev[slot], od[slot]
+__global__ void __launch_bounds__(kBlock) k_eo_split(EoArgs e, SpinorSet<QCD_HIP_F64> full, int f0,
+                                                     SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
+                                                     const int *ids) {
+  const int b = ids[blockIdx.y];
+  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= e.Vh) return;
+  Cx<double> v[12];
+  full.load(f0 + b, eo_site(h, 0, e.d).s, v);
+  ev.store(b, h, v);
+  full.load(f0 + b, eo_site(h, 1, e.d).s, v);
+  od.store(b, h, v);
+}
+
+// x[f0 + slot] += (ev[slot] on even sites, od[slot] on odd sites)
+__global__ void __launch_bounds__(kBlock) k_eo_accumulate(EoArgs e, SpinorSet<QCD_HIP_F64> x, int f0,
+                                                          SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
+                                                          const int *ids) {
+  const int b = ids[blockIdx.y];
+  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= e.Vh) return;
+#pragma unroll
+  for (unsigned p = 0; p < 2; p++) {
+    const unsigned s = eo_site(h, p, e.d).s;
+    Cx<double> xv[12], d[12];
+    x.load(f0 + b, s, xv);
+    (p ? od : ev).load(b, h, d);
+#pragma unroll
+    for (int k = 0; k < 12; k++) xv[k] = Cx<double>{xv[k].re + d[k].re, xv[k].im + d[k].im};
+    x.store(f0 + b, s, xv);
+  }
+}
+
+// out[slot] = M^-1 in[slot] (FP64 half vectors)
+__global__ void __launch_bounds__(kBlock) k_eo_minv(EoArgs e, SpinorSet<QCD_HIP_F64> in, SpinorSet<QCD_HIP_F64> out,
+                                                    const int *ids) {
+  const int b = ids[blockIdx.y];
+  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= e.Vh) return;
+  Cx<double> v[12];
+  in.load(b, h, v);
+  eo_mass<true>(v, e);
+  out.store(b, h, v);
+}
+
+// out[slot] = f_slot * in[slot] with f_slot = scale[slot] (FP64 result)
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_eo_scale64(EoArgs e, SpinorSet<P> in, SpinorSet<QCD_HIP_F64> out,
+                                                       const double *scale, size_t scale_stride, const int *ids) {
+  using R = typename Prec<P>::R;
+  const int b = ids[blockIdx.y];
+  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= e.Vh) return;
+  const double f = scale[(size_t)b * scale_stride];
+  Cx<R> v[12];
+  in.load(b, h, v);
+  Cx<double> o[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) o[k] = Cx<double>{f * (double)v[k].re, f * (double)v[k].im};
+  out.store(b, h, o);
+}
+
+EoArgs make_eo_args(qcd_hip_lattice full, double m, double tm, unsigned par) {
+  EoArgs e;
+  std::memset(&e, 0, sizeof(e));
+  e.d = make_args(full, m, tm);
+  e.Vh = (unsigned)(full->V / 2);
+  e.par = par;
+  const double a = m + 4.0, den = a * a + tm * tm;
+  e.mi_re = a / den;
+  e.mi_im = -tm / den;
+  return e;
+}
+
+bool eo_supported(qcd_hip_lattice lat) {
+  for (int d = 0; d < 4; d++)
+    if (lat->L[d] % 2) return false;
+  return true;
+}
+
+qcd_hip_lattice eo_half_lattice(qcd_hip_lattice full) {
+  auto *h = new qcd_hip_lattice_s(*full);
+  h->L[0] = full->L[0] / 2;
+  h->V = full->V / 2;
+  h->stride = round_up(h->V, 64);
+  return h;
+}
+
+template <int PS, int PG>
+static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
+                        int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st) {
+  const SpinorIn<PS> si = SpinorIn<PS>::of(in, ri);
+  const SpinorIn<PS> sb = b ? SpinorIn<PS>::of(b, rb) : si;
+  const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
+  const GaugeIn<PG> gu = GaugeIn<PG>::of(g);
+#define QH_E(E, RD) k_eo_hop<PS, PG, E, RD><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
+  switch (epi * 2 + (red ? 1 : 0)) {
+    case 0: QH_E(EO_MINV, false); break;
+    case 2: QH_E(EO_SCHUR, false); break;
+    case 4: QH_E(EO_SUB, false); break;
+    case 5: QH_E(EO_SUB, true); break;
+    case 6: QH_E(EO_SUB_MINV, false); break;
+    default: set_error("eo: unsupported epilogue %d (reduction %d)", epi, (int)red); return 1;
+  }
+#undef QH_E
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Batched EO hop: output sites of parity par; out/in/b are half-lattice
+// fields, g the full-lattice gauge field; row y of the batch works on vectors
+// ro + ids[y]*out_rs, ri + ids[y]*in_rs, rb + ids[y]*b_rs.  With red, |out|^2
+// partials go to partials[y * grid + x] and *grid_out receives grid.
+int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
+              qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m, double tm,
+              unsigned par, double *partials, hipStream_t st, int *grid_out) {
+  EoArgs e = make_eo_args(g->lat, m, tm, par);
+  e.d.ids = ids;
+  e.d.count = (unsigned)count;
+  e.d.in_rs = (unsigned)in_rs;
+  e.d.out_rs = (unsigned)out_rs;
+  e.d.b_rs = (unsigned)b_rs;
+  const int grid = grid_for(e.Vh);
+  if (grid_out) *grid_out = grid;
+  const dim3 gr((unsigned)grid, (unsigned)count);
+  const int ps = in->prec, pg = g->prec;
+  if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
+    return eo_launch_pp<QCD_HIP_F64, QCD_HIP_F64>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+  if (ps == QCD_HIP_F32 && pg == QCD_HIP_F32)
+    return eo_launch_pp<QCD_HIP_F32, QCD_HIP_F32>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+  if (ps == QCD_HIP_F16 && pg == QCD_HIP_F16)
+    return eo_launch_pp<QCD_HIP_F16, QCD_HIP_F16>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+  set_error("eo: unsupported precision pair (spinor %d, gauge %d)", ps, pg);
+  return 1;
+}
+
+int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
+             hipStream_t st) {
+  const EoArgs e = make_eo_args(full->lat, 0.0, 0.0, 0);
+  k_eo_split<<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(full), f0,
+                                                              SpinorSet<QCD_HIP_F64>::of(ev),
+                                                              SpinorSet<QCD_HIP_F64>::of(od), ids);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int eo_accumulate(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
+                  hipStream_t st) {
+  const EoArgs e = make_eo_args(x->lat, 0.0, 0.0, 0);
+  k_eo_accumulate<<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(x), f0,
+                                                                   SpinorSet<QCD_HIP_F64>::of(ev),
+                                                                   SpinorSet<QCD_HIP_F64>::of(od), ids);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int eo_minv(qcd_hip_lattice full, qcd_hip_field in, qcd_hip_field out, double m, double tm, const int *ids, int count,
+            hipStream_t st) {
+  const EoArgs e = make_eo_args(full, m, tm, 0);
+  k_eo_minv<<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(in),
+                                                             SpinorSet<QCD_HIP_F64>::of(out), ids);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int eo_scale64(qcd_hip_lattice full, qcd_hip_field in, qcd_hip_field out, const double *scale, size_t scale_stride,
+               const int *ids, int count, hipStream_t st) {
+  const EoArgs e = make_eo_args(full, 0.0, 0.0, 0);
+  const dim3 gr(grid_for(e.Vh), count);
+  switch (in->prec) {
+    case QCD_HIP_F64:
+      k_eo_scale64<QCD_HIP_F64><<<gr, kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(in),
+                                                       SpinorSet<QCD_HIP_F64>::of(out), scale, scale_stride, ids);
+      break;
+    case QCD_HIP_F32:
+      k_eo_scale64<QCD_HIP_F32><<<gr, kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F32>::of(in),
+                                                       SpinorSet<QCD_HIP_F64>::of(out), scale, scale_stride, ids);
+      break;
+    default:
+      k_eo_scale64<QCD_HIP_F16><<<gr, kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F16>::of(in),
+                                                       SpinorSet<QCD_HIP_F64>::of(out), scale, scale_stride, ids);
+  }
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcdhip
+
+using namespace qcdhip;
+
+extern "C" {
+
+int qcd_hip_dslash_schur(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, double m, double tm) {
+  QH_TRY(ensure_init());
+  if (!out || !in || !gauge || out->kind != QCD_HIP_SPINOR || in->kind != QCD_HIP_SPINOR ||
+      gauge->kind != QCD_HIP_GAUGE || out == in) {
+    set_error("dslash_schur: expected (spinor out, spinor in, gauge), out not aliasing in");
+    return 1;
+  }
+  const qcd_hip_lattice full = gauge->lat;
+  bool ok = eo_supported(full) && in->lat->V * 2 == full->V && out->lat->V == in->lat->V &&
+            in->lat->L[0] * 2 == full->L[0] && out->prec == in->prec && gauge->prec == in->prec &&
+            out->nrhs == in->nrhs;
+  for (int d = 1; d < 4; d++) ok = ok && in->lat->L[d] == full->L[d];
+  if (!ok) {
+    set_error("dslash_schur: in/out must be fields of the half lattice {T/2, X, Y, Z} of an all-even gauge "
+              "lattice, with equal precisions and rhs counts");
+    return 1;
+  }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  hipStream_t st = ctx()->stream;
+  qcd_hip_field te = nullptr;
+  QH_TRY(qcd_hip_spinor_create(&te, in->lat, in->prec, in->nrhs));
+  int rc = eo_launch(EO_MINV, false, te, 0, 1, in, 0, 1, gauge, nullptr, 0, 0, nullptr, in->nrhs, m, tm, 0, nullptr,
+                     st, nullptr);
+  if (!rc)
+    rc = eo_launch(EO_SCHUR, false, out, 0, 1, te, 0, 1, gauge, in, 0, 1, nullptr, in->nrhs, m, tm, 1, nullptr, st,
+                   nullptr);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("dslash_schur: %s", hipGetErrorString(hipGetLastError()));
+    rc = 2;
+  }
+  qcd_hip_field_destroy(te);
+  return rc;
+}
+
+}  // extern "C"

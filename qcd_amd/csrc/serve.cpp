@@ -64,6 +64,7 @@ int run_request(const qcd_hip_request &q, qcd_hip_result *res, double *solution,
   p.inner_prec = q.inner_prec;
   p.batch = q.nrhs;
   p.max_outer = 20000;
+  p.eo = lat && eo_supported(lat) ? 1 : 0;   // even-odd preconditioned inner solve when the extents allow it
   if (!rc) rc = qcd_hip_solver_create(&S, lat, &p);
   if (!rc) rc = qcd_hip_solver_set_gauge(S, g);
   qcd_hip_solver_stats st;

@@ -30,6 +30,21 @@
 
 namespace qcdhip {
 
+// eo.hip
+enum { EO_MINV = 0, EO_SCHUR = 1, EO_SUB = 2, EO_SUB_MINV = 3 };
+int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
+              qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m, double tm,
+              unsigned par, double *partials, hipStream_t st, int *grid_out);
+int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
+             hipStream_t st);
+int eo_accumulate(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
+                  hipStream_t st);
+int eo_minv(qcd_hip_lattice full, qcd_hip_field in, qcd_hip_field out, double m, double tm, const int *ids, int count,
+            hipStream_t st);
+int eo_scale64(qcd_hip_lattice full, qcd_hip_field in, qcd_hip_field out, const double *scale, size_t scale_stride,
+               const int *ids, int count, hipStream_t st);
+qcd_hip_lattice eo_half_lattice(qcd_hip_lattice full);
+
 constexpr int kN = kMaxKrylov;
 
 struct GcrScal {
@@ -380,7 +395,14 @@ struct qcd_hip_solver_s {
   int *ids_host = nullptr;          // pinned mirror
   double *part = nullptr;
   size_t part_bytes = 0, rowsz = 0;
-  unsigned nb = 0;                   // site-kernel workgroups
+  unsigned nb = 0;                   // site-kernel workgroups (full lattice)
+  // even-odd preconditioning (prm.eo): the inner GCR runs on the Schur
+  // complement D^ over odd-site half vectors
+  bool use_q = true;                 // outer system Q x = i g5 b (else D x = b)
+  qcd_hip_lattice half = nullptr;    // half lattice of the inner vectors (owned)
+  unsigned Vi = 0, nbi = 0;          // inner vector sites and site-kernel workgroups
+  qcd_hip_field te = nullptr;        // EO: B inner even temporaries of the matvec
+  qcd_hip_field re64 = nullptr, ro64 = nullptr, t64 = nullptr, eo64 = nullptr;   // EO: B FP64 half vectors each
 };
 
 namespace {
@@ -388,14 +410,22 @@ namespace {
 template <int P>
 int gcr_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream_t st) {
   const int n = S->prm.n_restart;
-  const unsigned V = (unsigned)S->lat->V, nb = S->nb;
-  const int epi = S->prm.op == QCD_HIP_OP_Q ? EPI_Q : EPI_D;
+  const unsigned V = S->Vi, nb = S->nbi;
+  const int epi = S->use_q ? EPI_Q : EPI_D;
   const double m = 1.0 / (2.0 * kappa) - 4.0;
   SpinorSet<P> Rs = SpinorSet<P>::of(S->Rs), Ws = SpinorSet<P>::of(S->Ws), E = SpinorSet<P>::of(S->E);
   const dim3 g2(nb, (unsigned)count);
   for (int k = 0; k < n; k++) {
-    QH_TRY(dslash_launch_batch(epi, false, S->Ws, k, n, S->Rs, k, n, S->gin, nullptr, 0, 0, S->ids, count, m, mu,
-                               nullptr, st, nullptr));
+    if (S->prm.eo) {
+      // w_k = D^ res_k = M res_k - H_oe (M^-1 H_eo res_k)
+      QH_TRY(eo_launch(EO_MINV, false, S->te, 0, 1, S->Rs, k, n, S->gin, nullptr, 0, 0, S->ids, count, m, mu, 0,
+                       nullptr, st, nullptr));
+      QH_TRY(eo_launch(EO_SCHUR, false, S->Ws, k, n, S->te, 0, 1, S->gin, S->Rs, k, n, S->ids, count, m, mu, 1,
+                       nullptr, st, nullptr));
+    } else {
+      QH_TRY(dslash_launch_batch(epi, false, S->Ws, k, n, S->Rs, k, n, S->gin, nullptr, 0, 0, S->ids, count, m, mu,
+                                 nullptr, st, nullptr));
+    }
     if (k > 0) {
       k_multidot<P><<<g2, kBlock, 0, st>>>(Ws, n, k, S->ids, V, S->part, S->rowsz);
       k_scal_dots<<<count, 1024, 0, st>>>(S->sc, S->part, S->rowsz, nb, k, n, S->ids);
@@ -412,19 +442,39 @@ int gcr_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream
 
 template <int P>
 int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
-  const dim3 g2(S->nb, (unsigned)count);
-  const unsigned V = (unsigned)S->lat->V;
-  k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc, SpinorSet<P>::of(S->Rs),
+  const dim3 g2(S->nbi, (unsigned)count);
+  const unsigned V = S->Vi;
+  if (!S->prm.eo) {
+    k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc, SpinorSet<P>::of(S->Rs),
+                                            S->prm.n_restart, S->ids, V);
+    QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
+    k_accumulate<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(x), x0, SpinorSet<P>::of(S->E), S->sc, S->ids,
+                                           V);
+    QH_CHECK(hipGetLastError());
+    return 0;
+  }
+  // Even-odd: D e = r  <=>  D^ e_o = r_o - H_oe M^-1 r_e,  e_e = M^-1 (r_e - H_eo e_o)
+  const double m = 1.0 / (2.0 * kappa) - 4.0;
+  int grid = 0;
+  QH_TRY(eo_split(S->r64, 0, S->re64, S->ro64, S->ids, count, st));
+  QH_TRY(eo_minv(S->lat, S->re64, S->t64, m, mu, S->ids, count, st));
+  QH_TRY(eo_launch(EO_SUB, true, S->eo64, 0, 1, S->t64, 0, 1, S->g64, S->ro64, 0, 1, S->ids, count, m, mu, 1, S->part,
+                   st, &grid));
+  k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);   // |r^_o| for the scaling
+  k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->eo64), S->sc, SpinorSet<P>::of(S->Rs),
                                           S->prm.n_restart, S->ids, V);
-  QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
-  k_accumulate<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(x), x0, SpinorSet<P>::of(S->E), S->sc, S->ids, V);
   QH_CHECK(hipGetLastError());
+  QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
+  QH_TRY(eo_scale64(S->lat, S->E, S->eo64, &S->sc[0].rnorm, sizeof(GcrScal) / sizeof(double), S->ids, count, st));
+  QH_TRY(eo_launch(EO_SUB_MINV, false, S->t64, 0, 1, S->eo64, 0, 1, S->g64, S->re64, 0, 1, S->ids, count, m, mu, 0,
+                   nullptr, st, nullptr));
+  QH_TRY(eo_accumulate(x, x0, S->t64, S->eo64, S->ids, count, st));
   return 0;
 }
 
 // r_b = bq_b - A x_{x0+b} for the active slots; residual norms to rn_host
 int true_residual(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
-  const int epi = S->prm.op == QCD_HIP_OP_Q ? EPI_RES_Q : EPI_RES_D;
+  const int epi = S->use_q ? EPI_RES_Q : EPI_RES_D;
   int grid = 0;
   QH_TRY(dslash_launch_batch(epi, true, S->r64, 0, 1, x, x0, 1, S->g64, S->bq, 0, 1, S->ids, count,
                              1.0 / (2.0 * kappa) - 4.0, mu, S->part, st, &grid));
@@ -447,8 +497,7 @@ int solve_batch(qcd_hip_solver_s *S, qcd_hip_field x, qcd_hip_field b, int x0, i
                 hipStream_t st, qcd_hip_solver_stats *tot, int *worst) {
   const unsigned V = (unsigned)S->lat->V;
   k_start<<<dim3(S->nb, (unsigned)cnt), kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(b), SpinorSet<QCD_HIP_F64>::of(x),
-                                                         x0, SpinorSet<QCD_HIP_F64>::of(S->bq),
-                                                         S->prm.op == QCD_HIP_OP_Q, V);
+                                                         x0, SpinorSet<QCD_HIP_F64>::of(S->bq), S->use_q, V);
   QH_CHECK(hipGetLastError());
   QH_CHECK(hipMemsetAsync(S->sc, 0, sizeof(GcrScal) * S->B, st));
   std::vector<int> act(cnt);
@@ -506,6 +555,7 @@ void qcd_hip_solver_default_params(qcd_hip_solver_params *p) {
   p->op = QCD_HIP_OP_Q;
   p->keep_q1 = 0;
   p->batch = 1;
+  p->eo = 0;
 }
 
 int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hip_solver_params *prm) {
@@ -513,8 +563,12 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   QH_TRY(ensure_init());
   if (!lat || !prm || prm->n_restart < 1 || prm->n_restart > kN || prm->tol <= 0.0 || prm->max_outer < 1 ||
       prm->inner_prec < QCD_HIP_F64 || prm->inner_prec > QCD_HIP_F16 || prm->batch < 1 || prm->batch > 1024 ||
-      (prm->op != QCD_HIP_OP_Q && prm->op != QCD_HIP_OP_D)) {
-    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1, batch 1..1024)", kN);
+      (prm->op != QCD_HIP_OP_Q && prm->op != QCD_HIP_OP_D) || (prm->eo != 0 && prm->eo != 1)) {
+    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1, batch 1..1024, eo 0/1)", kN);
+    return 1;
+  }
+  if (prm->eo && !eo_supported(lat)) {
+    set_error("solver_create: even-odd preconditioning needs every lattice extent even");
     return 1;
   }
   auto *S = new qcd_hip_solver_s();
@@ -527,10 +581,18 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   }
   S->B = prm->batch;
   S->nb = (unsigned)grid_for(lat->V);
+  S->use_q = prm->op == QCD_HIP_OP_Q && !prm->eo;   // the EO inner system is D's Schur complement
+  qcd_hip_lattice ilat = lat;
+  if (prm->eo) ilat = S->half = eo_half_lattice(lat);
+  S->Vi = (unsigned)ilat->V;
+  S->nbi = (unsigned)grid_for(ilat->V);
   const int n = prm->n_restart, B = S->B;
-  int rc = qcd_hip_spinor_create(&S->Rs, lat, prm->inner_prec, n * B);
-  if (!rc) rc = qcd_hip_spinor_create(&S->Ws, lat, prm->inner_prec, n * B);
-  if (!rc) rc = qcd_hip_spinor_create(&S->E, lat, prm->inner_prec, B);
+  int rc = qcd_hip_spinor_create(&S->Rs, ilat, prm->inner_prec, n * B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->Ws, ilat, prm->inner_prec, n * B);
+  if (!rc) rc = qcd_hip_spinor_create(&S->E, ilat, prm->inner_prec, B);
+  if (!rc && prm->eo) rc = qcd_hip_spinor_create(&S->te, ilat, prm->inner_prec, B);
+  for (qcd_hip_field *f : {&S->re64, &S->ro64, &S->t64, &S->eo64})
+    if (!rc && prm->eo) rc = qcd_hip_spinor_create(f, ilat, QCD_HIP_F64, B);
   if (!rc) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, B);
   if (!rc) rc = qcd_hip_spinor_create(&S->bq, lat, QCD_HIP_F64, B);
   if (!rc) rc = dev_alloc((void **)&S->sc, sizeof(GcrScal) * B);
@@ -561,6 +623,7 @@ void qcd_hip_solver_destroy(qcd_hip_solver S) {
   qcd_hip_field_destroy(S->E);
   qcd_hip_field_destroy(S->r64);
   qcd_hip_field_destroy(S->bq);
+  for (qcd_hip_field f : {S->te, S->re64, S->ro64, S->t64, S->eo64}) qcd_hip_field_destroy(f);
   if (S->gin && S->gin != S->g64) qcd_hip_field_destroy(S->gin);
   if (S->stream) {
     (void)hipStreamSynchronize(S->stream);
@@ -572,6 +635,7 @@ void qcd_hip_solver_destroy(qcd_hip_solver S) {
   dev_free(S->part, S->part_bytes);
   if (S->rn_host) (void)hipHostFree(S->rn_host);
   if (S->ids_host) (void)hipHostFree(S->ids_host);
+  delete S->half;
   delete S;
 }
 
@@ -618,7 +682,7 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
   hipStream_t st = S->stream;
   const auto t0 = std::chrono::steady_clock::now();
   // Tune the inner and outer operators before the first timed use.
-  QH_TRY(dslash_prepare(S->Ws, S->Rs, S->gin));
+  if (!S->prm.eo) QH_TRY(dslash_prepare(S->Ws, S->Rs, S->gin));
   QH_TRY(dslash_prepare(S->r64, x, S->g64));
   qcd_hip_solver_stats tot;
   std::memset(&tot, 0, sizeof(tot));

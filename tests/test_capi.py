@@ -135,3 +135,18 @@ def test_library_without_gpu_fails_loudly():
     lat = ctypes.c_void_p()
     rc = Q.lib().qcd_hip_init(0)
     assert rc != 0 and b"device" in Q.lib().qcd_hip_last_error().lower()
+
+
+def test_eo_split_merge_round_trip():
+    """Half-lattice order used by the even-odd solver: site l of parity p is entry l >> 1."""
+    L = (6, 4, 2, 8)
+    V = int(np.prod(L))
+    a = np.arange(V * 12, dtype=np.float64).reshape(V, 4, 3)
+    ev, od = Q.eo_split(a, L)
+    assert ev.shape == od.shape == (V // 2, 4, 3)
+    assert np.array_equal(Q.eo_merge(ev, od, L), a)
+    T, X, Y = L[0], L[1], L[2]
+    for l in range(V):
+        t, x, y, z = l % T, (l // T) % X, (l // (T * X)) % Y, l // (T * X * Y)
+        half = ev if (t + x + y + z) % 2 == 0 else od
+        assert np.array_equal(half[l >> 1], a[l])
