@@ -90,7 +90,10 @@ __device__ __forceinline__ void group_sum(double (&v)[NV], double *out, unsigned
 // Krylov vector j of slot b is vector b*n + j of the Rs / Ws fields; row y
 // writes its partial sums at part + y*rowsz, laid out [value][workgroup].
 
-// d_j = <w_j, w_k> for j < k
+// d_j = <w_j, w_k> for j < k, in chunks of kDotChunk basis vectors: their
+// loads are in flight together and the workgroup reduces the chunk's sums
+// at once (each d_j is summed exactly as it would be alone)
+constexpr int kDotChunk = 4;
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_multidot(SpinorSet<P> W, int n, int k, const int *ids, unsigned V,
                                                      double *part, size_t rowsz) {
@@ -101,18 +104,26 @@ __global__ void __launch_bounds__(kBlock) k_multidot(SpinorSet<P> W, int n, int 
   const bool ok = s < V;
   Cx<R> wk[12];
   if (ok) W.load(b * n + k, s, wk);
-  for (int j = 0; j < k; j++) {
-    double v[2] = {0.0, 0.0};
-    if (ok) {
-      Cx<R> wj[12];
-      W.load(b * n + j, s, wj);
+  for (int j0 = 0; j0 < k; j0 += kDotChunk) {
+    double v[2 * kDotChunk];
 #pragma unroll
-      for (int c = 0; c < 12; c++) {
-        v[0] += (double)wj[c].re * (double)wk[c].re + (double)wj[c].im * (double)wk[c].im;
-        v[1] += (double)wj[c].re * (double)wk[c].im - (double)wj[c].im * (double)wk[c].re;
+    for (int i = 0; i < 2 * kDotChunk; i++) v[i] = 0.0;
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < kDotChunk; i++) {
+        if (j0 + i < k) {
+          Cx<R> wj[12];
+          W.load(b * n + j0 + i, s, wj);
+#pragma unroll
+          for (int c = 0; c < 12; c++) {
+            v[2 * i] += (double)wj[c].re * (double)wk[c].re + (double)wj[c].im * (double)wk[c].im;
+            v[2 * i + 1] += (double)wj[c].re * (double)wk[c].im - (double)wj[c].im * (double)wk[c].re;
+          }
+        }
       }
     }
-    group_sum<2>(v, pr + (size_t)2 * j * gridDim.x, gridDim.x, blockIdx.x);
+    // sums of j >= k are written too (zeros) but never read
+    group_sum<2 * kDotChunk>(v, pr + (size_t)2 * j0 * gridDim.x, gridDim.x, blockIdx.x);
   }
 }
 
@@ -135,6 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, Spino
       Cx<double> acc[12];
 #pragma unroll
       for (int c = 0; c < 12; c++) acc[c] = Cx<double>{(double)wk[c].re, (double)wk[c].im};
+#pragma unroll 4
       for (int j = 0; j < k; j++) {
         Cx<R> wj[12];
         W.load(b * n + j, s, wj);
@@ -197,6 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, con
   Cx<double> acc[12];
 #pragma unroll
   for (int c = 0; c < 12; c++) acc[c] = Cx<double>{0.0, 0.0};
+#pragma unroll 4
   for (int j = 0; j < n; j++) {
     Cx<R> r[12];
     Rs.load(b * n + j, s, r);
@@ -602,7 +615,8 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   if (!rc && hipHostMalloc((void **)&S->ids_host, sizeof(int) * B, hipHostMallocDefault) != hipSuccess) rc = 4;
   // per batch row: 2(n-1) multi-dot sums per site workgroup, or one sum per
   // Dslash workgroup (a z-march grid never exceeds site workgroups + T*Z)
-  const size_t rowmax = std::max<size_t>((size_t)2 * n + 4, 4) * ((size_t)S->nb + 64);
+  // (the multi-dot writes whole chunks: up to 2 (n + kDotChunk) sums)
+  const size_t rowmax = ((size_t)2 * (n + kDotChunk) + 4) * ((size_t)S->nb + 64);
   const size_t dsl = (size_t)dslash_grid_bound(lat);
   S->rowsz = std::max(rowmax, dsl);
   S->part_bytes = sizeof(double) * S->rowsz * B;
