@@ -94,6 +94,10 @@ void qcd_addVector3d(qcd_vector *sum, qcd_vector *summand1, qcd_vector *summand2
 qcd_complex_16 qcd_mulAdjointVectorVector(qcd_vector *factor1, qcd_vector *factor2);
 /* sqrt of the sum over all 24V reals of x^2 */
 qcd_real_8 qcd_normVector(qcd_vector *vec);
+/* spatial links projected onto SU(3) in place: the unitary polar factor times
+ * exp(-i arg(det)/3) (include/qcd_blas.h:271, lib/src/qcd_blas.c:940-1364);
+ * runs on the GPU */
+void qcd_projectSU33d(qcd_gaugeField *gf);
 
 #ifdef __cplusplus
 }

@@ -161,6 +161,12 @@ int  qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field gauge64, dou
 int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
                         double kappa, double mu, double *rel);
 
+/* nsteps of 3d APE smearing (lib/src/qcd_smearing.c:205-273) of the FP64
+ * gauge field in into out (distinct fields on one lattice): spatial links ->
+ * P_SU3[U + alpha * spatial staples], temporal links copied.  nsteps = 0
+ * copies; alpha = 0 projects the spatial links (qcd_blas.c:940-1364). */
+int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nsteps);
+
 /* Even-odd Schur complement D^ = M - H_oe M^-1 H_eo (M = m + 4 + i tm g5, H
  * the hopping term of D) applied to every rhs of in: in and out hold the odd
  * sites of a lattice {T, X, Y, Z} as fields of its half lattice

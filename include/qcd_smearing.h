@@ -21,6 +21,10 @@ int qcd_gaussIteration3d(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, qcd
 /* gaugeCom selects whether the reference exchanges gauge faces; with one rank
  * it has no effect. */
 int qcd_gaussIteration3dAll(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, qcd_uint_2 gaugeCom);
+/* One step of 3d APE smearing (include/qcd_smearing.h:41, lib/src/qcd_smearing.c:205-273):
+ * spatial links -> P_SU3[ U + alpha * sum of the 4 spatial staples ], temporal
+ * links copied.  Runs on the GPU; apeu may alias u. */
+int qcd_apeSmear3d(qcd_gaugeField *apeu, qcd_gaugeField *u, qcd_real_8 alpha);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,8 @@ def lib():
             "orc_spinor_gaussian": (None, [_dp, lg, ctypes.c_uint64]),
             "orc_point_source": (None, [_dp, ctypes.POINTER(ctypes.c_int), i, i, i, i, i, i]),
             "orc_plaquette": (d, [vp, _dp]),
+            "orc_project_su3": (None, [_dp]),
+            "orc_ape3d": (None, [vp, _dp, _dp, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -189,3 +191,20 @@ def point_source(L, site, spin, col):
 
 def plaquette(geo, u):
     return lib().orc_plaquette(geo._h, _p(u))
+
+
+def project_su3(m):
+    """SU(3) projection (lib/src/qcd_blas.c:940-1364) of every 3x3 matrix of m (..., 3, 3)."""
+    a = np.array(m, dtype=np.complex128, order="C").reshape(-1, 3, 3)
+    for k in range(a.shape[0]):
+        blk = np.ascontiguousarray(a[k])
+        lib().orc_project_su3(_p(blk))
+        a[k] = blk
+    return a.reshape(np.shape(m))
+
+
+def ape3d(geo, u, alpha):
+    """One 3d APE smearing step (lib/src/qcd_smearing.c:205-273); returns the new field."""
+    out = np.empty_like(u)
+    lib().orc_ape3d(geo._h, _p(out), _p(np.ascontiguousarray(u)), float(alpha))
+    return out

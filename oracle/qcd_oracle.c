@@ -704,3 +704,140 @@ double orc_plaquette(const orc_geo *g, const double *u)
    }
    return sum / ((double)g->V * 3.0 * 6.0);
 }
+
+
+/* ------------------------------------------------------------ APE smearing */
+
+static cplx cdivc(cplx a, cplx b)
+{
+   const double d = b.re * b.re + b.im * b.im;
+   cplx r = {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+   return r;
+}
+
+static cplx det3(const cplx a[3][3])
+{
+   cplx d = {0.0, 0.0};
+   int c;
+   for (c = 0; c < 3; c++) {   /* expansion along row 0, cyclic minors carry their sign */
+      cplx t = cmul(a[1][(c + 1) % 3], a[2][(c + 2) % 3]);
+      const cplx s = cmul(a[1][(c + 2) % 3], a[2][(c + 1) % 3]);
+      t.re -= s.re;
+      t.im -= s.im;
+      d = cadd(d, cmul(a[0][c], t));
+   }
+   return d;
+}
+
+/* (a^-1)^+ by the adjugate: inv[r][c] = cof[c][r] / det, so
+ * (a^-1)^+[r][c] = conj(cof[r][c] / det) */
+static void inv_dagger3(cplx out[3][3], const cplx a[3][3])
+{
+   const cplx d = det3(a);
+   int r, c;
+   for (r = 0; r < 3; r++)
+      for (c = 0; c < 3; c++) {
+         const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+         cplx cof = cmul(a[r1][c1], a[r2][c2]);
+         const cplx s = cmul(a[r1][c2], a[r2][c1]);
+         cof.re -= s.re;
+         cof.im -= s.im;
+         out[r][c] = conjc(cdivc(cof, d));
+      }
+}
+
+static double frob2(const cplx a[3][3])
+{
+   double s = 0.0;
+   int r, c;
+   for (r = 0; r < 3; r++)
+      for (c = 0; c < 3; c++) s += a[r][c].re * a[r][c].re + a[r][c].im * a[r][c].im;
+   return s;
+}
+
+/* The reference (qcd_blas.c:950-1364) diagonalises H^2 = M^+ M by Cardano's
+ * formula and forms M H^-1 exp(-i arg(det M)/3).  The polar factor is unique
+ * for non-singular M, so it is computed here by Newton's iteration
+ * X <- (g X + X^-+ / g) / 2 with Frobenius-norm scaling g (Higham 1986),
+ * which converges quadratically from X = M. */
+void orc_project_su3(double *m18)
+{
+   cplx M[3][3], X[3][3], Y[3][3];
+   int r, c, it;
+   memcpy(M, m18, sizeof(M));
+   memcpy(X, M, sizeof(X));
+   for (it = 0; it < 100; it++) {
+      double g = 1.0, diff = 0.0;
+      inv_dagger3(Y, X);
+      if (it < 8) g = sqrt(sqrt(frob2(Y) / frob2(X)));
+      for (r = 0; r < 3; r++)
+         for (c = 0; c < 3; c++) {
+            cplx n = {0.5 * (g * X[r][c].re + Y[r][c].re / g), 0.5 * (g * X[r][c].im + Y[r][c].im / g)};
+            diff += (n.re - X[r][c].re) * (n.re - X[r][c].re) + (n.im - X[r][c].im) * (n.im - X[r][c].im);
+            X[r][c] = n;
+         }
+      if (it >= 8 && diff <= 1e-30) break;
+   }
+   {
+      const cplx d = det3(M);
+      const double ph = atan2(d.im, d.re) / 3.0;
+      const cplx e = {cos(ph), -sin(ph)};
+      for (r = 0; r < 3; r++)
+         for (c = 0; c < 3; c++) X[r][c] = cmul(X[r][c], e);
+   }
+   memcpy(m18, X, sizeof(X));
+}
+
+/* apeu_mu(x) = P_SU3[ U_mu(x) + alpha sum_{nu != mu, spatial} (forward staple +
+ * backward staple) ] for mu = 1, 2, 3; apeu_0 = U_0 (qcd_smearing.c:205-273).
+ * Staples are summed in the reference's order: all forward ones
+ * U_nu(x) U_mu(x+nu) U_nu(x+mu)^+, then all backward ones
+ * U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu). */
+void orc_ape3d(const orc_geo *g, double *out, const double *u, double alpha)
+{
+   long l;
+#pragma omp parallel for schedule(static)
+   for (l = 0; l < g->V; l++) {
+      int mu, nu, r, c;
+      memcpy(out + 72 * l, u + 72 * l, 18 * sizeof(double));
+      for (mu = 1; mu < 4; mu++) {
+         cplx S[3][3], T[3][3], P[3][3], A[3][3], M[3][3];
+         memset(S, 0, sizeof(S));
+         for (nu = 1; nu < 4; nu++) {
+            if (nu == mu) continue;
+            const cplx (*Un)[3] = (const cplx (*)[3])(u + 72 * l + 18 * nu);
+            const cplx (*Um)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * l + nu] + 18 * mu);
+            const cplx (*Ua)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * l + mu] + 18 * nu);
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) A[r][c] = conjc(Ua[c][r]);
+            mat_mul3(T, Un, Um);
+            mat_mul3(P, T, A);
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) S[r][c] = cadd(S[r][c], P[r][c]);
+         }
+         for (nu = 1; nu < 4; nu++) {
+            if (nu == mu) continue;
+            const long lm = g->minus[4 * l + nu];
+            const cplx (*Un)[3] = (const cplx (*)[3])(u + 72 * lm + 18 * nu);
+            const cplx (*Um)[3] = (const cplx (*)[3])(u + 72 * lm + 18 * mu);
+            const cplx (*Ua)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * lm + mu] + 18 * nu);
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) A[r][c] = conjc(Un[c][r]);
+            mat_mul3(T, Um, Ua);
+            mat_mul3(P, A, T);
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) S[r][c] = cadd(S[r][c], P[r][c]);
+         }
+         {
+            const cplx (*U0)[3] = (const cplx (*)[3])(u + 72 * l + 18 * mu);
+            for (r = 0; r < 3; r++)
+               for (c = 0; c < 3; c++) {
+                  M[r][c].re = U0[r][c].re + alpha * S[r][c].re;
+                  M[r][c].im = U0[r][c].im + alpha * S[r][c].im;
+               }
+         }
+         orc_project_su3((double *)M);
+         memcpy(out + 72 * l + 18 * mu, M, sizeof(M));
+      }
+   }
+}

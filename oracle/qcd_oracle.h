@@ -70,6 +70,14 @@ void orc_spinor_gaussian(double *v, long V, uint64_t seed);
 void orc_point_source(double *v, const int L[4], int t, int x, int y, int z, int spin, int col);
 double orc_plaquette(const orc_geo *g, const double *u);
 
+/* Projection of one 3x3 complex matrix (18 doubles, row-major) onto SU(3) in
+ * place: the unitary polar factor of M times exp(-i arg(det M) / 3)
+ * (lib/src/qcd_blas.c:940-1364). */
+void orc_project_su3(double *m);
+/* One step of 3d APE smearing of every spatial link, temporal links copied
+ * (lib/src/qcd_smearing.c:205-273); out must not alias u. */
+void orc_ape3d(const orc_geo *g, double *out, const double *u, double alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,7 @@ def lib():
         "qcd_hip_spinor_gaussian_host": (i, [_dp, lg, ctypes.c_uint64]),
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
         "qcd_hip_dslash_schur": (i, [_vp, _vp, _vp, d, d]),
+        "qcd_hip_ape_smear": (i, [_vp, _vp, d, i]),
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
         "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
         "qcd_hip_dslash_set_config": (i, [_vp, _vp, i, i]),
@@ -158,6 +159,8 @@ def lib():
         "qcd_hip_bindGaugeField": (i, [pf]),
         "qcd_hip_unbindGaugeField": (None, []),
         "qcd_gaussIteration3d": (i, [pf, pf, d, ctypes.c_uint32]),
+        "qcd_apeSmear3d": (i, [pf, pf, d]),
+        "qcd_projectSU33d": (None, [pf]),
         "qcd_gaussIteration3dAll": (i, [pf, pf, d, ctypes.c_uint16]),
         "qcd_zeroVector": (None, [pf]),
         "qcd_copyVector": (None, [pf, pf]),
@@ -338,6 +341,11 @@ def dslash(out, inp, gauge, m, tm=0.0, op=OP_D):
 def dslash_schur(out, inp, gauge, m, tm=0.0):
     """Even-odd Schur complement D^ = M - H_oe M^-1 H_eo on odd-site half-lattice fields."""
     _check(lib().qcd_hip_dslash_schur(out.h, inp.h, gauge.h, float(m), float(tm)), "qcd_hip_dslash_schur")
+
+
+def ape_smear(out, inp, alpha, nsteps=1):
+    """nsteps of 3d APE smearing of the FP64 gauge field inp into out (lib/src/qcd_smearing.c:205-273)."""
+    _check(lib().qcd_hip_ape_smear(out.h, inp.h, float(alpha), int(nsteps)), "qcd_hip_ape_smear")
 
 
 def eo_split(a, L):

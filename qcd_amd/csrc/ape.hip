@@ -1,0 +1,197 @@
+// ape.hip -- 3D APE smearing of the gauge field with SU(3) projection.
+//
+// One step (lib/src/qcd_smearing.c:205-273), for the spatial directions
+// mu = 1, 2, 3 (temporal links are copied unchanged):
+//   U'_mu(x) = P_SU3[ U_mu(x) + alpha sum_{nu = 1..3, nu != mu}
+//                 ( U_nu(x) U_mu(x+nu) U_nu(x+mu)^+ + U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) ) ]
+// P_SU3(M) is the unitary polar factor of M, M (M^+ M)^-1/2, times
+// exp(-i arg(det M) / 3) (lib/src/qcd_blas.c:940-1364, which diagonalises
+// M^+ M by Cardano's formula).  The polar factor is unique for non-singular
+// M; it is computed here by Newton's iteration X <- (g X + X^-+ / g) / 2 with
+// Frobenius-norm scaling g, which converges quadratically from X = M.
+// One lane per (site, direction); FP64 throughout.  The gauge field is read
+// in the device layout (one array per direction of [tile][9][64] links).
+#include "blas_common.h"
+
+namespace qcdhip {
+
+using C = Cx<double>;
+
+__device__ __forceinline__ C cm(C a, C b) { return C{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)}; }
+__device__ __forceinline__ C cj(C a) { return C{a.re, -a.im}; }
+
+// c = a b (DA: a^+, DB: b^+)
+template <bool DA, bool DB>
+__device__ __forceinline__ void mm3(C c[9], const C a[9], const C b[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C s{0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const C x = DA ? cj(a[3 * k + i]) : a[3 * i + k];
+        const C y = DB ? cj(b[3 * j + k]) : b[3 * k + j];
+        const C p = cm(x, y);
+        s = C{s.re + p.re, s.im + p.im};
+      }
+      c[3 * i + j] = s;
+    }
+}
+
+__device__ __forceinline__ C det3(const C a[9]) {
+  C d{0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {   // expansion along row 0, cyclic minors carry their sign
+    const C t = cm(a[3 + (c + 1) % 3], a[6 + (c + 2) % 3]);
+    const C s = cm(a[3 + (c + 2) % 3], a[6 + (c + 1) % 3]);
+    d = C{d.re + a[c].re * (t.re - s.re) - a[c].im * (t.im - s.im), d.im + a[c].re * (t.im - s.im) + a[c].im * (t.re - s.re)};
+  }
+  return d;
+}
+
+// SU(3) projection in place
+__device__ void project_su3(C m[9]) {
+  C x[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) x[k] = m[k];
+  for (int it = 0; it < 100; it++) {
+    // y = (x^-1)^+ = conj(cofactor(x)) / conj(det x)
+    const C d = det3(x);
+    const double dd = d.re * d.re + d.im * d.im;
+    C y[9];
+    double nx = 0.0, ny = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const C t = cm(x[3 * r1 + c1], x[3 * r2 + c2]);
+        const C s = cm(x[3 * r1 + c2], x[3 * r2 + c1]);
+        const C cof{t.re - s.re, t.im - s.im};
+        // cof / det, then conjugate
+        const C q{(cof.re * d.re + cof.im * d.im) / dd, (cof.im * d.re - cof.re * d.im) / dd};
+        y[3 * r + c] = cj(q);
+        nx += x[3 * r + c].re * x[3 * r + c].re + x[3 * r + c].im * x[3 * r + c].im;
+        ny += q.re * q.re + q.im * q.im;
+      }
+    const double g = it < 8 ? sqrt(sqrt(ny / nx)) : 1.0;
+    double diff = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const C n{0.5 * (g * x[k].re + y[k].re / g), 0.5 * (g * x[k].im + y[k].im / g)};
+      diff += (n.re - x[k].re) * (n.re - x[k].re) + (n.im - x[k].im) * (n.im - x[k].im);
+      x[k] = n;
+    }
+    if (it >= 8 && diff <= 1e-30) break;
+  }
+  const C d = det3(m);
+  const double ph = atan2(d.im, d.re) / 3.0;
+  const C e{cos(ph), -sin(ph)};
+#pragma unroll
+  for (int k = 0; k < 9; k++) m[k] = cm(x[k], e);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ape3d(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ out,
+                                                  size_t stride, double alpha) {
+  const unsigned j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (j >= 4 * a.V) return;
+  const unsigned s = j / 4, mu = j % 4;
+  const Site st = site_coords(s, a);
+  C m[9];
+  U.load(mu, s, m);
+  if (mu != 0) {
+    C S[9], A[9], B[9], T[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
+    // forward staples U_nu(x) U_mu(x+nu) U_nu(x+mu)^+, then backward ones
+    // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) (the reference's order)
+    for (int pass = 0; pass < 2; pass++)
+      for (unsigned nu = 1; nu < 4; nu++) {
+        if (nu == mu) continue;
+        unsigned sn, sm, sa;
+        if (pass == 0) {
+          sn = s;
+          sm = nu == 1 ? neighbour<1, +1>(st, a) : nu == 2 ? neighbour<2, +1>(st, a) : neighbour<3, +1>(st, a);
+          sa = mu == 1 ? neighbour<1, +1>(st, a) : mu == 2 ? neighbour<2, +1>(st, a) : neighbour<3, +1>(st, a);
+        } else {
+          sn = nu == 1 ? neighbour<1, -1>(st, a) : nu == 2 ? neighbour<2, -1>(st, a) : neighbour<3, -1>(st, a);
+          const Site sb = site_coords(sn, a);
+          sm = sn;
+          sa = mu == 1 ? neighbour<1, +1>(sb, a) : mu == 2 ? neighbour<2, +1>(sb, a) : neighbour<3, +1>(sb, a);
+        }
+        U.load(nu, sn, A);
+        U.load(mu, sm, B);
+        C W[9];
+        U.load(nu, sa, W);
+        if (pass == 0) {
+          mm3<false, false>(T, A, B);
+          mm3<false, true>(B, T, W);
+        } else {
+          mm3<false, false>(T, B, W);
+          mm3<true, false>(B, A, T);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
+      }
+#pragma unroll
+    for (int k = 0; k < 9; k++) m[k] = C{m[k].re + alpha * S[k].re, m[k].im + alpha * S[k].im};
+    project_su3(m);
+  }
+  double2 *o = out + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
+#pragma unroll
+  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
+}
+
+int ape_step(qcd_hip_field out, qcd_hip_field in, double alpha, hipStream_t st) {
+  const DslashArgs a = make_args(in->lat, 0.0, 0.0);
+  k_ape3d<<<grid_for(4 * in->lat->V), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                                        (size_t)in->lat->stride, alpha);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcdhip
+
+using namespace qcdhip;
+
+extern "C" {
+
+int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nsteps) {
+  QH_TRY(ensure_init());
+  if (!out || !in || out == in || out->kind != QCD_HIP_GAUGE || in->kind != QCD_HIP_GAUGE ||
+      out->prec != QCD_HIP_F64 || in->prec != QCD_HIP_F64 || out->lat->V != in->lat->V || nsteps < 0) {
+    set_error("ape_smear: expected two distinct FP64 gauge fields on one lattice and nsteps >= 0");
+    return 1;
+  }
+  for (int d = 0; d < 4; d++)
+    if (out->lat->L[d] != in->lat->L[d]) {
+      set_error("ape_smear: lattice extents differ");
+      return 1;
+    }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  hipStream_t st = ctx()->stream;
+  if (nsteps == 0) {
+    QH_CHECK(hipMemcpyAsync(out->data, in->data, out->bytes, hipMemcpyDeviceToDevice, st));
+    QH_CHECK(hipStreamSynchronize(st));
+    return 0;
+  }
+  qcd_hip_field tmp = nullptr;
+  if (nsteps > 1) QH_TRY(qcd_hip_gauge_create(&tmp, in->lat, QCD_HIP_F64));
+  // ping-pong so that the last step writes out
+  qcd_hip_field src = in;
+  int rc = 0;
+  for (int k = 0; k < nsteps && !rc; k++) {
+    qcd_hip_field dst = ((nsteps - 1 - k) % 2 == 0) ? out : tmp;
+    rc = ape_step(dst, src, alpha, st);
+    src = dst;
+  }
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("ape_smear: %s", hipGetErrorString(hipGetLastError()));
+    rc = 2;
+  }
+  qcd_hip_field_destroy(tmp);
+  return rc;
+}
+
+}  // extern "C"

@@ -368,6 +368,38 @@ int qcd_gaussIteration3dAll(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, 
   return report("qcd_gaussIteration3dAll", rc);
 }
 
+// lib/src/qcd_smearing.c:205-273 (one step; apeu may alias u here)
+int qcd_apeSmear3d(qcd_gaugeField *apeu, qcd_gaugeField *u, qcd_real_8 alpha) {
+  if (!u || !u->initialized || !apeu || !apeu->initialized) {
+    fprintf(stderr, "Error in qcd_apeSmear3d! Gauge fields must be properly initialized\n");
+    return 1;
+  }
+  qcd_hip_lattice lat;
+  qcd_hip_field g, out = nullptr;
+  int rc = lattice_for(u->geo, &lat);
+  if (!rc) rc = gauge_for(u, lat, &g);
+  if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
+  if (!rc) rc = qcd_hip_ape_smear(out, g, alpha, 1);
+  if (!rc) rc = qcd_hip_gauge_download(out, (double *)apeu->D);
+  qcd_hip_field_destroy(out);
+  return report("qcd_apeSmear3d", rc);
+}
+
+// lib/src/qcd_blas.c:940-1364: spatial links projected onto SU(3) in place
+// (an APE step with alpha = 0)
+void qcd_projectSU33d(qcd_gaugeField *gf) {
+  if (!gf || !gf->initialized) return;
+  qcd_hip_lattice lat;
+  qcd_hip_field g, out = nullptr;
+  int rc = lattice_for(gf->geo, &lat);
+  if (!rc) rc = gauge_for(gf, lat, &g);
+  if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
+  if (!rc) rc = qcd_hip_ape_smear(out, g, 0.0, 1);
+  if (!rc) rc = qcd_hip_gauge_download(out, (double *)gf->D);
+  qcd_hip_field_destroy(out);
+  report("qcd_projectSU33d", rc);
+}
+
 // ============================================================ BLAS
 // host data movement: lib/src/qcd_blas.c:137-233
 void qcd_zeroVector(qcd_vector *vec) { std::memset(vec->D, 0, 12 * vec->geo->lV * sizeof(qcd_complex_16)); }

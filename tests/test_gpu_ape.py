@@ -1,0 +1,85 @@
+"""3d APE smearing and the spatial SU(3) projection on the GPU
+(lib/src/qcd_smearing.c:205-273, lib/src/qcd_blas.c:940-1364) against the
+CPU oracle, whose projection and staples are pinned in tests/test_oracle.py."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = (16, 8, 8, 8)
+
+
+def test_ape_steps_match_oracle(gpu):
+    Q = gpu
+    u = Q.gauge_generate(L, 0.3, 701)
+    geo = O.Geometry(L)
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat).upload(u)
+    out = Q.Gauge(lat)
+    ref1 = O.ape3d(geo, u, 0.5)
+    Q.ape_smear(out, g, 0.5, 1)
+    assert np.abs(out.download() - ref1).max() <= 1e-13
+    ref3 = O.ape3d(geo, O.ape3d(geo, ref1, 0.5), 0.5)
+    Q.ape_smear(out, g, 0.5, 3)
+    assert np.abs(out.download() - ref3).max() <= 1e-13
+    # the input field is left unchanged, and zero steps copy it
+    assert np.array_equal(g.download(), u)
+    Q.ape_smear(out, g, 0.5, 0)
+    assert np.array_equal(out.download(), u)
+
+
+def test_projection_of_perturbed_links(gpu):
+    """alpha = 0: spatial links projected, temporal links copied bit for bit."""
+    Q = gpu
+    rng = np.random.default_rng(7)
+    u = Q.gauge_generate(L, 0.3, 702)
+    m = u + 0.2 * (rng.normal(size=u.shape) + 1j * rng.normal(size=u.shape))
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat).upload(m)
+    out = Q.Gauge(lat)
+    Q.ape_smear(out, g, 0.0, 1)
+    got = out.download()
+    assert np.array_equal(got[:, 0], m[:, 0])
+    ref = O.project_su3(m[:, 1:])
+    assert np.abs(got[:, 1:] - ref).max() <= 1e-13
+
+
+def test_dropin_ape_and_projection(gpu):
+    Q = gpu
+    Ls = (8, 4, 4, 4)
+    u = Q.gauge_generate(Ls, 0.4, 703)
+    di = Q.DropIn(Ls)
+    try:
+        lib = Q.lib()
+        uu, aa = di.gauge(u), di.gauge()
+        assert lib.qcd_apeSmear3d(ctypes.byref(aa), ctypes.byref(uu), 0.5) == 0
+        ref = O.ape3d(O.Geometry(Ls), u, 0.5)
+        assert np.abs(di.view(aa, 36).reshape(u.shape) - ref).max() <= 1e-13
+        # in place (apeu aliasing u) works here
+        assert lib.qcd_apeSmear3d(ctypes.byref(uu), ctypes.byref(uu), 0.5) == 0
+        assert np.abs(di.view(uu, 36).reshape(u.shape) - ref).max() <= 1e-13
+        rng = np.random.default_rng(9)
+        m = u + 0.1 * rng.normal(size=u.shape)
+        pp = di.gauge(m)
+        lib.qcd_projectSU33d(ctypes.byref(pp))
+        got = di.view(pp, 36).reshape(u.shape)
+        assert np.array_equal(got[:, 0], m[:, 0])
+        assert np.abs(got[:, 1:] - O.project_su3(m[:, 1:])).max() <= 1e-13
+    finally:
+        di.close()
+
+
+def test_ape_rejects_bad_arguments(gpu):
+    Q = gpu
+    lat = Q.Lattice((8, 4, 4, 4))
+    g = Q.Gauge(lat)
+    with pytest.raises(Q.QcdError):
+        Q.ape_smear(g, g, 0.5, 1)
+    with pytest.raises(Q.QcdError):
+        Q.ape_smear(Q.Gauge(lat, Q.F32), g, 0.5, 1)
+    with pytest.raises(Q.QcdError):
+        Q.ape_smear(Q.Gauge(lat), g, 0.5, -1)

@@ -297,3 +297,94 @@ def test_generator_su3_and_matches_product():
     assert np.array_equal(Q.spinor_gaussian(L, 23), O.spinor_gaussian(L, 23))
     # eps -> 0 gives the unit field
     assert np.abs(O.gauge_generate(L, 0.0, 24) - unit_gauge(int(np.prod(L)))).max() == 0.0
+
+
+# ---------------------------------------------------------------- APE smearing
+def polar_su3(m):
+    """Unitary polar factor (by SVD) times exp(-i arg(det m)/3): an independent construction."""
+    w, _, vh = np.linalg.svd(m)
+    return (w @ vh) * np.exp(-1j * np.angle(np.linalg.det(m)) / 3)
+
+
+def spatial_plaquette(L, u):
+    """Average Re tr / 3 of the spatial plaquettes, by loops over sites (brute force)."""
+    T, X, Y, Z = L
+    V = T * X * Y * Z
+    c = coords(L)
+    step = [1, T, T * X, T * X * Y]
+
+    def shift(l, mu):
+        cc = c[l].copy()
+        cc[mu] = (cc[mu] + 1) % L[mu]
+        return int(cc[0] + T * (cc[1] + X * (cc[2] + Y * cc[3])))
+
+    tot = 0.0
+    for l in range(V):
+        for mu in range(1, 4):
+            for nu in range(mu + 1, 4):
+                p = u[l, mu] @ u[shift(l, mu), nu] @ u[shift(l, nu), mu].conj().T @ u[l, nu].conj().T
+                tot += np.trace(p).real / 3
+    return tot / (3 * V)
+
+
+def test_su3_projection_is_the_polar_factor():
+    rng = np.random.default_rng(11)
+    m = rng.normal(size=(20, 3, 3)) + 1j * rng.normal(size=(20, 3, 3))
+    p = O.project_su3(m)
+    for k in range(20):
+        assert np.abs(p[k] - polar_su3(m[k])).max() <= 1e-13
+        assert np.abs(p[k].conj().T @ p[k] - np.eye(3)).max() <= 1e-14
+        assert abs(np.linalg.det(p[k]) - 1) <= 1e-14
+    # a positive multiple of an SU(3) matrix projects onto itself
+    g = random_su3(4, 12)
+    assert np.abs(O.project_su3(3.7 * g) - g).max() <= 1e-14
+
+
+def test_ape_fixed_point_temporal_links_and_smoothing():
+    L = (4, 4, 4, 4)
+    geo = O.Geometry(L)
+    V = geo.V
+    one = unit_gauge(V)
+    assert np.abs(O.ape3d(geo, one, 0.5) - one).max() <= 1e-15
+    u = O.gauge_generate(L, 0.6, 21)
+    a = O.ape3d(geo, u, 0.5)
+    assert np.array_equal(a[:, 0], u[:, 0])   # temporal links untouched
+    for l in range(0, V, 7):
+        for mu in range(1, 4):
+            assert np.abs(a[l, mu].conj().T @ a[l, mu] - np.eye(3)).max() <= 1e-14
+            assert abs(np.linalg.det(a[l, mu]) - 1) <= 1e-14
+    p0, p1 = spatial_plaquette(L, u), spatial_plaquette(L, a)
+    p3 = spatial_plaquette(L, O.ape3d(geo, O.ape3d(geo, a, 0.5), 0.5))
+    assert p0 < p1 < p3 <= 1.0
+
+
+def test_ape_staples_brute_force_and_gauge_covariance():
+    L = (2, 4, 2, 4)
+    geo = O.Geometry(L)
+    T, X, Y, Z = L
+    V = geo.V
+    u = O.gauge_generate(L, 0.8, 31)
+    alpha = 0.4
+    c = coords(L)
+
+    def nb(l, mu, d):
+        cc = c[l].copy()
+        cc[mu] = (cc[mu] + d) % L[mu]
+        return int(cc[0] + T * (cc[1] + X * (cc[2] + Y * cc[3])))
+
+    a = O.ape3d(geo, u, alpha)
+    for l in range(V):
+        for mu in range(1, 4):
+            s = np.zeros((3, 3), dtype=complex)
+            for nu in range(1, 4):
+                if nu == mu:
+                    continue
+                s += u[l, nu] @ u[nb(l, nu, 1), mu] @ u[nb(l, mu, 1), nu].conj().T
+                b = nb(l, nu, -1)
+                s += u[b, nu].conj().T @ u[b, mu] @ u[nb(b, mu, 1), nu]
+            assert np.abs(a[l, mu] - polar_su3(u[l, mu] + alpha * s)).max() <= 1e-13
+    # covariance: smearing the transformed field = transforming the smeared field
+    g = random_su3(V, 32)
+    up = np.ascontiguousarray(gauge_transform(L, u, g))
+    ap = O.ape3d(geo, up, alpha)
+    assert np.abs(ap - gauge_transform(L, a, g)).max() <= 1e-12
