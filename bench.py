@@ -155,6 +155,51 @@ def config3_sites(L=(48, 24, 24, 24), seed=3003):
     return [tuple(int(rng.integers(0, L[d])) for d in range(4)) for _ in range(4)]
 
 
+def config5_leg(Q, batch=4, seed=5005):
+    """48^3x96, eps 0.3, kappa 0.132: 10 APE steps (alpha 0.5) of the links, 12
+    point sources at the origin Gauss-smeared (alpha 4.0, 50 iterations, FP64)
+    with the APE links, then a 12-rhs mixed solve (FP16 inner, even-odd) with
+    the original links to ||b - D x|| / ||b|| <= 1e-10, checked in FP64."""
+    L, kappa = (96, 48, 48, 48), 0.132
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat)
+    Q.gauge_generate_device(g, 0.3, seed)
+    ga = Q.Gauge(lat)
+    Q.synchronize()
+    t0 = time.perf_counter()
+    Q.ape_smear(ga, g, 0.5, 10)
+    Q.synchronize()
+    t_ape = time.perf_counter() - t0
+    b = Q.Spinor(lat, nrhs=12)
+    for r in range(12):
+        Q.point_source_device(b, r, (0, 0, 0, 0), r // 3, r % 3)
+    Q.synchronize()
+    t0 = time.perf_counter()
+    Q.gauss_smear(b, ga, 4.0, 0, 50)
+    Q.synchronize()
+    t_smear = time.perf_counter() - t0
+    del ga
+    x = Q.Spinor(lat, nrhs=12)
+    S = Q.Solver(lat, g, n_restart=32, tol=1e-10, inner_prec="f16", batch=batch, max_outer=20000, eo=1)
+    b1 = Q.Spinor(lat).upload(Q.point_source(L, (0, 0, 0, 0), 0, 0))
+    x1 = Q.Spinor(lat)
+    Q.Solver(lat, g, n_restart=32, tol=1e-2, inner_prec="f16", batch=1, eo=1).solve(x1, b1, kappa)
+    del b1, x1
+    Q.synchronize()
+    t0 = time.perf_counter()
+    st = S.solve(x, b, kappa)
+    Q.synchronize()
+    t_solve = time.perf_counter() - t0
+    worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
+    assert worst <= 1e-10, f"config 5 solve not converged: {worst}"
+    return {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
+                      "Gauss-smeared (alpha 4.0, 50 iterations, FP64), mixed FP16-inner even-odd solve, tol 1e-10",
+            "ape_seconds": t_ape, "smear_seconds": t_smear, "solve_seconds": t_solve,
+            "solves_per_s": 12 / t_solve, "batch": batch, "gcr_iterations": st["inner_iters"],
+            "outer_cycles": st["outer"], "worst_b_minus_Dx": worst,
+            "pipeline_seconds": t_ape + t_smear + t_solve}
+
+
 def serve_stream(n, rate, seed):
     """Poisson arrivals; volumes, kappa, eps and rhs counts drawn per request."""
     rng = np.random.default_rng(seed)
@@ -228,6 +273,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-config5", action="store_true", help="skip the 48^3x96 APE + smearing + solve pipeline")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the headline kernel's HBM traffic")
     args = ap.parse_args()
@@ -285,6 +331,11 @@ def main():
             log(f"solves config3 {name}: {solves['config3'][name]['solves_per_s']:.2f}/s")
         solves["config3"]["speedup_mixed_vs_fp64"] = (solves["config3"]["mixed_f16_eo_batched48"]["solves_per_s"] /
                                                       solves["config3"]["fp64_eo_one_at_a_time"]["solves_per_s"])
+        if not args.no_config5:
+            solves["config5"] = config5_leg(Q, batch=6)
+            c5 = solves["config5"]
+            log(f"config5 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
+                f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s)")
 
     serve = None
     if not args.no_serve:

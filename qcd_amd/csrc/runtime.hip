@@ -109,9 +109,18 @@ static int stage(size_t bytes, void **out) {
 }
 
 // ------------------------------------------------------------ scratch fields
-// Library-owned spinors reused across calls (resized when a larger one is
-// needed), so that entry points needing temporaries do not allocate per call.
-static qcd_hip_field g_scratch[8] = {nullptr};
+// Library-owned spinors reused across calls (reallocated when the shape
+// changes), so that entry points needing temporaries do not allocate per call.
+// A slot outlives the lattices it is used with, so it records the shape it
+// was allocated for (plane stride, precision, rhs count) and never looks at
+// the lattice it was last lent to, which the caller may have destroyed since.
+struct ScratchSlot {
+  qcd_hip_field f = nullptr;
+  long stride = 0;
+  int prec = -1;
+  int nrhs = 0;
+};
+static ScratchSlot g_scratch[8];
 
 int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_field *out) {
   std::lock_guard<std::recursive_mutex> lock(state_mutex());
@@ -119,17 +128,19 @@ int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_fi
     set_error("scratch slot %d out of range", slot);
     return 1;
   }
-  qcd_hip_field f = g_scratch[slot];
-  if (f && (f->prec != prec || f->nrhs < nrhs || f->lat->stride != lat->stride)) {
-    qcd_hip_field_destroy(f);
-    g_scratch[slot] = f = nullptr;
+  ScratchSlot &sc = g_scratch[slot];
+  if (sc.f && (sc.prec != prec || sc.nrhs < nrhs || sc.stride != lat->stride)) {
+    qcd_hip_field_destroy(sc.f);
+    sc = ScratchSlot();
   }
-  if (!f) {
-    QH_TRY(qcd_hip_spinor_create(&f, lat, prec, nrhs));
-    g_scratch[slot] = f;
+  if (!sc.f) {
+    QH_TRY(qcd_hip_spinor_create(&sc.f, lat, prec, nrhs));
+    sc.stride = lat->stride;
+    sc.prec = prec;
+    sc.nrhs = nrhs;
   }
-  f->lat = lat;
-  *out = f;
+  sc.f->lat = lat;   // lent for this call; the caller's lattice is alive while it uses the field
+  *out = sc.f;
   return 0;
 }
 

@@ -109,3 +109,21 @@ def test_f16_store_of_denormal_site_maxima(gpu):
     y = Q.Spinor(lat, Q.F16)
     Q.dslash(y, f, g16, 1 / (2 * 0.13) - 4, 0.0, Q.OP_Q)
     assert np.isfinite(y.download()).all()
+
+
+def test_scratch_fields_outlive_destroyed_lattices(gpu):
+    """b_minus_Dx borrows a library scratch spinor.  After a smaller lattice (and its
+    fields) is destroyed, a larger lattice must get a scratch buffer of its own size."""
+    Q = gpu
+    import gc
+    for Ls in ((8, 4, 4, 4), (16, 8, 8, 8), (8, 4, 4, 4), (16, 8, 8, 12)):
+        u = Q.gauge_generate(Ls, 0.3, 901)
+        x, b = Q.spinor_gaussian(Ls, 902), Q.spinor_gaussian(Ls, 903)
+        lat = Q.Lattice(Ls)
+        g = Q.Gauge(lat).upload(u)
+        fx, fb = Q.Spinor(lat).upload(x), Q.Spinor(lat).upload(b)
+        res = Q.b_minus_Dx(fx, fb, g, 0.13)
+        ref = O.b_minus_Dx(O.Geometry(Ls), x, b, u, 0.13)
+        assert abs(res - ref) <= 1e-12 * ref
+        del g, fx, fb, lat
+        gc.collect()
