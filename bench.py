@@ -339,6 +339,7 @@ def main():
 
     serve = None
     if not args.no_serve:
+        Q.lib().qcd_hip_release_cache()   # the serving process starts without the other legs' cached blocks
         reqs = serve_stream(args.serve_requests, args.serve_rate, 777)
         serve = {"stream": f"{args.serve_requests} requests, Poisson rate {args.serve_rate}/s, volumes "
                            "{8^3x16,16^3x32,24^3x48}, kappa U[0.120,0.132], eps U[0.2,0.35], rhs {1,12}, "

@@ -40,10 +40,17 @@ int ensure_init() {
 
 // Caching allocator: freed blocks are kept (up to kPoolCap bytes) and reused
 // for requests of the same size.  hipFree synchronises the whole device, so
-// freeing inside a serving loop would serialise concurrent requests.
+// freeing inside a serving loop would serialise concurrent requests.  When
+// the pool is full the least recently freed blocks are returned to the
+// runtime, so a workload that changes shapes keeps the blocks it now uses.
 constexpr size_t kPoolCap = (size_t)64 << 30;
-static std::multimap<size_t, void *> g_pool;
+struct PoolBlock {
+  void *p;
+  uint64_t tick;   // when it was freed
+};
+static std::multimap<size_t, PoolBlock> g_pool;
 static size_t g_pool_bytes = 0;
+static uint64_t g_pool_tick = 0;
 
 int dev_alloc(void **p, size_t bytes) {
   *p = nullptr;
@@ -52,7 +59,7 @@ int dev_alloc(void **p, size_t bytes) {
     std::lock_guard<std::recursive_mutex> lock(state_mutex());
     auto it = g_pool.find(bytes);
     if (it != g_pool.end()) {
-      *p = it->second;
+      *p = it->second.p;
       g_pool.erase(it);
       g_pool_bytes -= bytes;
       g_ctx.bytes_in_use += bytes;
@@ -80,15 +87,21 @@ int dev_alloc(void **p, size_t bytes) {
 void dev_free(void *p, size_t bytes) {
   if (!p) return;
   g_ctx.bytes_in_use -= bytes;
-  {
-    std::lock_guard<std::recursive_mutex> lock(state_mutex());
-    if (g_pool_bytes + bytes <= kPoolCap) {
-      g_pool.emplace(bytes, p);
-      g_pool_bytes += bytes;
-      return;
-    }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  if (bytes > kPoolCap) {
+    (void)hipFree(p);
+    return;
   }
-  (void)hipFree(p);
+  while (g_pool_bytes + bytes > kPoolCap && !g_pool.empty()) {   // evict the oldest blocks
+    auto old = g_pool.begin();
+    for (auto it = g_pool.begin(); it != g_pool.end(); ++it)
+      if (it->second.tick < old->second.tick) old = it;
+    (void)hipFree(old->second.p);
+    g_pool_bytes -= old->first;
+    g_pool.erase(old);
+  }
+  g_pool.emplace(bytes, PoolBlock{p, ++g_pool_tick});
+  g_pool_bytes += bytes;
 }
 
 // Device staging buffer for AoS transfers, grown on demand.
@@ -321,7 +334,7 @@ size_t qcd_hip_bytes_in_use(void) { return g_ctx.bytes_in_use.load(); }
 
 void qcd_hip_release_cache(void) {
   std::lock_guard<std::recursive_mutex> lock(state_mutex());
-  for (auto &b : g_pool) (void)hipFree(b.second);
+  for (auto &b : g_pool) (void)hipFree(b.second.p);
   g_pool.clear();
   g_pool_bytes = 0;
 }
