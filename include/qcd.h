@@ -168,5 +168,7 @@ void qcd_waitall(qcd_geometry *geo);
 #include <qcd_blas.h>
 #include <qcd_wilson.h>
 #include <qcd_smearing.h>
+#include <qcd_io.h>
+#include <qcd_observables.h>
 
 #endif

@@ -167,6 +167,9 @@ int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hi
  * copies; alpha = 0 projects the spatial links (qcd_blas.c:940-1364). */
 int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nsteps);
 
+/* mean plaquette of an FP64 gauge field (lib/src/qcd_observables.c:15-36) */
+int qcd_hip_plaquette(qcd_hip_field gauge64, double *plaq);
+
 /* Even-odd Schur complement D^ = M - H_oe M^-1 H_eo (M = m + 4 + i tm g5, H
  * the hopping term of D) applied to every rhs of in: in and out hold the odd
  * sites of a lattice {T, X, Y, Z} as fields of its half lattice

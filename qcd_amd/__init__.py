@@ -123,6 +123,7 @@ def lib():
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
         "qcd_hip_dslash_schur": (i, [_vp, _vp, _vp, d, d]),
         "qcd_hip_ape_smear": (i, [_vp, _vp, d, i]),
+        "qcd_hip_plaquette": (i, [_vp, _dp]),
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
         "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
         "qcd_hip_dslash_set_config": (i, [_vp, _vp, i, i]),

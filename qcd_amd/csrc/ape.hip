@@ -143,6 +143,35 @@ __global__ void __launch_bounds__(kBlock) k_ape3d(DslashArgs a, GaugeIn<QCD_HIP_
   for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
 }
 
+// Plaquette sum of one site over the 6 planes mu < nu (qcd_observables.c:15-36):
+// Re tr[U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+], one workgroup partial each.
+__global__ void __launch_bounds__(kBlock) k_plaquette(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double *partials) {
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double sum = 0.0;
+  if (s < a.V) {
+    const Site st = site_coords(s, a);
+    unsigned fwd[4];
+    fwd[0] = neighbour<0, +1>(st, a);
+    fwd[1] = neighbour<1, +1>(st, a);
+    fwd[2] = neighbour<2, +1>(st, a);
+    fwd[3] = neighbour<3, +1>(st, a);
+    for (int mu = 0; mu < 3; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        C A[9], B[9], T[9], P[9];
+        U.load(mu, s, A);
+        U.load(nu, fwd[mu], B);
+        mm3<false, false>(T, A, B);
+        U.load(mu, fwd[nu], A);
+        mm3<false, true>(P, T, A);
+        U.load(nu, s, A);
+        // Re tr(P A^+) = sum_ij Re(P_ij conj(A_ij))
+#pragma unroll
+        for (int k = 0; k < 9; k++) sum += P[k].re * A[k].re + P[k].im * A[k].im;
+      }
+  }
+  block_reduce_store(sum, partials, blockIdx.x);
+}
+
 int ape_step(qcd_hip_field out, qcd_hip_field in, double alpha, hipStream_t st) {
   const DslashArgs a = make_args(in->lat, 0.0, 0.0);
   k_ape3d<<<grid_for(4 * in->lat->V), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
@@ -192,6 +221,33 @@ int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nst
   }
   qcd_hip_field_destroy(tmp);
   return rc;
+}
+
+int qcd_hip_plaquette(qcd_hip_field gauge, double *plaq) {
+  QH_TRY(ensure_init());
+  if (!gauge || !plaq || gauge->kind != QCD_HIP_GAUGE || gauge->prec != QCD_HIP_F64) {
+    set_error("plaquette: expected an FP64 gauge field");
+    return 1;
+  }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  hipStream_t st = ctx()->stream;
+  const DslashArgs a = make_args(gauge->lat, 0.0, 0.0);
+  const int grid = grid_for(gauge->lat->V);
+  double *part = nullptr;
+  QH_TRY(dev_alloc((void **)&part, sizeof(double) * (size_t)grid));
+  k_plaquette<<<grid, kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(gauge), part);
+  int rc = hipGetLastError() == hipSuccess ? 0 : 2;
+  if (!rc) rc = finalize(part, grid, 1, ctx()->scalars, st);
+  double sum = 0.0;
+  if (!rc) rc = hipMemcpyAsync(&sum, ctx()->scalars, sizeof(double), hipMemcpyDeviceToHost, st) ? 2 : 0;
+  if (!rc) rc = hipStreamSynchronize(st) ? 2 : 0;
+  dev_free(part, sizeof(double) * (size_t)grid);
+  if (rc) {
+    set_error("plaquette: kernel failed");
+    return rc;
+  }
+  *plaq = sum / ((double)gauge->lat->V * 3.0 * 6.0);
+  return 0;
 }
 
 }  // extern "C"

@@ -400,6 +400,22 @@ void qcd_projectSU33d(qcd_gaugeField *gf) {
   report("qcd_projectSU33d", rc);
 }
 
+// lib/src/qcd_observables.c:15-36
+qcd_real_8 qcd_calculatePlaquette(qcd_gaugeField *u) {
+  if (!u || !u->initialized) {
+    fprintf(stderr, "Error in qcd_calculatePlaquette! Gauge field must be properly initialized\n");
+    return 0.0;
+  }
+  qcd_hip_lattice lat;
+  qcd_hip_field g;
+  double plaq = 0.0;
+  int rc = lattice_for(u->geo, &lat);
+  if (!rc) rc = gauge_for(u, lat, &g);
+  if (!rc) rc = qcd_hip_plaquette(g, &plaq);
+  report("qcd_calculatePlaquette", rc);
+  return rc ? 0.0 : plaq;
+}
+
 // ============================================================ BLAS
 // host data movement: lib/src/qcd_blas.c:137-233
 void qcd_zeroVector(qcd_vector *vec) { std::memset(vec->D, 0, 12 * vec->geo->lV * sizeof(qcd_complex_16)); }

@@ -12,7 +12,7 @@ from conftest import ROOT
 
 import qcd_amd as Q
 
-HEADERS = ["qcd.h", "qcd_wilson.h", "qcd_smearing.h", "qcd_blas.h", "qcd_hip.h"]
+HEADERS = ["qcd.h", "qcd_wilson.h", "qcd_smearing.h", "qcd_blas.h", "qcd_io.h", "qcd_observables.h", "qcd_hip.h"]
 
 
 def declared_functions():
@@ -150,3 +150,20 @@ def test_eo_split_merge_round_trip():
         t, x, y, z = l % T, (l // T) % X, (l // (T * X)) % Y, l // (T * X * Y)
         half = ev if (t + x + y + z) % 2 == 0 else od
         assert np.array_equal(half[l >> 1], a[l])
+
+
+def test_drivers_fail_loudly_without_gpu(tmp_path):
+    """The invert driver has no CPU path: without a GPU it reports an error and fails."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    exe = os.path.join(ROOT, "build", "bin", "invert")
+    if not os.path.exists(exe):
+        pytest.skip("drivers not built")
+    ini = tmp_path / "invert.ini"
+    ini.write_text("<processors_txyz>1 1 1 1</processors_txyz>\n<lattice_txyz>4 4 4 4</lattice_txyz>\n"
+                   "<source_type>HMC_PROPAGATOR</source_type>\n<source>none</source>\n"
+                   "<solution_type>HMC_PROPAGATOR</solution_type>\n<solution>none</solution>\n"
+                   "<cfg_name>none</cfg_name>\n<N_restart>8</N_restart>\n<kappa>0.13</kappa>\n<mu>0</mu>\n")
+    r = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+    assert r.returncode != 0 and "Error" in r.stderr

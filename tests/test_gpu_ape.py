@@ -83,3 +83,26 @@ def test_ape_rejects_bad_arguments(gpu):
         Q.ape_smear(Q.Gauge(lat, Q.F32), g, 0.5, 1)
     with pytest.raises(Q.QcdError):
         Q.ape_smear(Q.Gauge(lat), g, 0.5, -1)
+
+
+def test_plaquette_device_and_dropin(gpu):
+    """qcd_calculatePlaquette (lib/src/qcd_observables.c:15-36) on the GPU against the oracle; 1 for U = 1."""
+    Q = gpu
+    u = Q.gauge_generate(L, 0.3, 704)
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat).upload(u)
+    got = ctypes.c_double()
+    assert Q.lib().qcd_hip_plaquette(g.h, ctypes.byref(got)) == 0
+    ref = O.plaquette(O.Geometry(L), u)
+    assert abs(got.value - ref) <= 1e-13
+    one = np.zeros_like(u)
+    one[:, :, [0, 1, 2], [0, 1, 2]] = 1.0
+    di = Q.DropIn(L)
+    try:
+        lib = Q.lib()
+        lib.qcd_calculatePlaquette.restype = ctypes.c_double
+        lib.qcd_calculatePlaquette.argtypes = [ctypes.POINTER(Q.QcdField)]
+        assert abs(lib.qcd_calculatePlaquette(ctypes.byref(di.gauge(u))) - ref) <= 1e-13
+        assert abs(lib.qcd_calculatePlaquette(ctypes.byref(di.gauge(one))) - 1.0) <= 1e-15
+    finally:
+        di.close()
