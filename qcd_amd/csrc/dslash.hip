@@ -21,6 +21,29 @@ namespace qcdhip {
 // SIMD; OCC & 16: scheduling fence between hop directions.
 #define QH_WAVES(OCC) __attribute__((amdgpu_waves_per_eu(((OCC) & 15) > 0 ? ((OCC) & 15) : 1, ((OCC) & 15) > 0 ? ((OCC) & 15) : 8)))
 
+// compile-time (direction, projector sign, dagger) of one hop
+template <int MU, int S, bool DAG>
+struct HopTag {
+  static constexpr int mu = MU, s = S;
+  static constexpr bool dag = DAG;
+};
+
+// acc = -2 (m + 4 + i tm g5) psi + the backward z hop (cz0, cz1) carried
+// from the previous z step; psi given by its stored values rc (norm nc)
+template <int PS, typename R>
+__device__ __forceinline__ void local_terms(Cx<R> acc[4][3], const typename Prec<PS>::T rc[12], float nc,
+                                            const Cx<R> cz0[3], const Cx<R> cz1[3], R mp4, R tm) {
+  Cx<R> psi[12];
+  SpinorIn<PS>::convert(rc, nc, psi);
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++) {
+    const R gg = sp < 2 ? tm : -tm;
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[sp][c] = diag_term(psi[3 * sp + c], mp4, gg);
+  }
+  rebuild_add<3, +1>(acc, cz0, cz1);
+}
+
 template <int PS, int PG, int EPI, bool RED, int OCC>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
@@ -37,9 +60,8 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
     // bits: acc = -2 diag(psi) + z back + z forward + t, x, y hops; out = -acc/2
     Cx<R> acc[4][3];
     const R mp4 = R(a.mp4), tm = R(a.tm);
-    const bool tw = a.twisted[3] != 0;
     {
-      Cx<R> psi[12], u[9], g0[3], g1[3];
+      Cx<R> psi[12];
       in.load(s, psi);
 #pragma unroll
       for (int sp = 0; sp < 4; sp++) {
@@ -51,14 +73,8 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
         }
       }
       const unsigned sb = neighbour<3, -1>(st, a), sf = neighbour<3, +1>(st, a);
-      in.load(sb, psi);
-      U.load(3, sb, u);
-      hop_half<3, +1, true>(g0, g1, psi, u, tw, Cx<R>{(R)a.ph_re[3], -(R)a.ph_im[3]});
-      rebuild_add<3, +1>(acc, g0, g1);
-      in.load(sf, psi);
-      U.load(3, s, u);
-      hop_half<3, -1, false>(g0, g1, psi, u, tw, Cx<R>{(R)a.ph_re[3], (R)a.ph_im[3]});
-      rebuild_add<3, -1>(acc, g0, g1);
+      hop_acc_at<3, +1, true>(acc, in, sb, U, sb, Cx<R>{(R)a.ph_re[3], -(R)a.ph_im[3]});
+      hop_acc_at<3, -1, false>(acc, in, sf, U, s, Cx<R>{(R)a.ph_re[3], (R)a.ph_im[3]});
     }
     constexpr bool F = (OCC & 16) != 0;
     hop_dir<0, PS, PG, R, F>(acc, a, st, in, U);
@@ -110,7 +126,6 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
     st.t = col - st.x * a.L[0];
     st.y = st.x / a.L[1];
     st.x -= st.y * a.L[1];
-    const bool tw = a.twisted[3] != 0;
     const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
     const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
     const R mp4 = R(a.mp4), tm = R(a.tm);
@@ -118,10 +133,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
     {
       const unsigned zm = z0 == 0 ? a.L[3] - 1 : z0 - 1;
       const unsigned sm = col + P3 * zm;
-      Cx<R> psi[12], u[9];
-      in.load(sm, psi);
-      U.load(3, sm, u);
-      hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+      hop_half_at<3, +1, true>(cz0, cz1, in, sm, U, sm, phm);
     }
     for (unsigned z = z0; z < z1; z++) {
       st.z = z;
@@ -129,24 +141,19 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
       const unsigned s = st.s;
       Cx<R> acc[4][3];
       {
-        Cx<R> psi[12], u[9], g0[3], g1[3];
-        in.load(s, psi);
-#pragma unroll
-        for (int sp = 0; sp < 4; sp++) {
-          const R gg = sp < 2 ? tm : -tm;
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const Cx<R> v = psi[3 * sp + c];
-            acc[sp][c] = diag_term(v, mp4, gg);
-          }
-        }
-        rebuild_add<3, +1>(acc, cz0, cz1);
-        U.load(3, s, u);
+        typename Prec<PS>::T rc[12];
+        typename Prec<PG>::T u[9];
+        float nc = 0.f;
+        in.load_raw(s, rc, nc);
+        local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
+        U.load_raw(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
-        hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+        hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
-        in.load(sf, psi);
-        hop_half<3, -1, false>(g0, g1, psi, u, tw, php);
+        Cx<R> g0[3], g1[3];
+        float nf = 0.f;
+        in.load_raw(sf, rc, nf);
+        hop_half_raw<3, -1, false, PS, PG>(g0, g1, rc, nf, u, php);
         rebuild_add<3, -1>(acc, g0, g1);
       }
       constexpr bool F = (OCC & 16) != 0;
@@ -220,7 +227,6 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   st.x = x0 + xr;
   st.y = y;
   const unsigned col = t + Lt * (st.x + Lx * y);
-  const bool tw = a.twisted[3] != 0;
   const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
   const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
   const R mp4 = R(a.mp4), tm = R(a.tm);
@@ -234,10 +240,7 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   if (active) {
     const unsigned zm = z0 == 0 ? a.L[3] - 1 : z0 - 1;
     const unsigned sm = col + P3 * zm;
-    Cx<R> psi[12], u[9];
-    in.load(sm, psi);
-    U.load(3, sm, u);
-    hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+    hop_half_at<3, +1, true>(cz0, cz1, in, sm, U, sm, phm);
   }
   for (unsigned z = z0; z < z1; z++) {   // z0, z1 are uniform over the workgroup
     st.z = z;
@@ -267,47 +270,40 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
     if (active) {
       Cx<R> acc[4][3];
       {
-        Cx<R> psi[12], u[9], g0[3], g1[3];
-        SpinorIn<PS>::convert(rc, nc, psi);
-#pragma unroll
-        for (int sp = 0; sp < 4; sp++) {
-          const R gg = sp < 2 ? tm : -tm;
-#pragma unroll
-          for (int c = 0; c < 3; c++) acc[sp][c] = diag_term(psi[3 * sp + c], mp4, gg);
-        }
-        rebuild_add<3, +1>(acc, cz0, cz1);
-        U.load(3, s, u);
+        typename Prec<PG>::T u[9];
+        local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
+        U.load_raw(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
-        hop_half<3, +1, true>(cz0, cz1, psi, u, tw, phm);
+        hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
-        in.load(sf, psi);
-        hop_half<3, -1, false>(g0, g1, psi, u, tw, php);
+        Cx<R> g0[3], g1[3];
+        float nf = 0.f;
+        in.load_raw(sf, rc, nf);
+        hop_half_raw<3, -1, false, PS, PG>(g0, g1, rc, nf, u, php);
         rebuild_add<3, -1>(acc, g0, g1);
       }
       constexpr bool F = (OCC & 16) != 0;
       // t and x hops from the staged rows
       {
-        auto staged = [&](unsigned slot, Cx<R> v[12]) {
+        // one hop of direction MU from staged slot `slot`, link at site sl
+        auto staged_hop = [&](auto mu_s_dag, unsigned slot, unsigned sl, Cx<R> ph) {
+          constexpr int MU = decltype(mu_s_dag)::mu, S = decltype(mu_s_dag)::s;
+          constexpr bool DAG = decltype(mu_s_dag)::dag;
           T r[12];
+          typename Prec<PG>::T u[9];
 #pragma unroll
           for (int k = 0; k < 12; k++) r[k] = stv[k * nslot + slot];
-          SpinorIn<PS>::convert(r, PS == QCD_HIP_F16 ? stn[slot] : 0.f, v);
+          U.load_raw(MU, sl, u);
+          Cx<R> g0[3], g1[3];
+          hop_half_raw<MU, S, DAG, PS, PG>(g0, g1, r, PS == QCD_HIP_F16 ? stn[slot] : 0.f, u, ph);
+          rebuild_add<MU, S>(acc, g0, g1);
         };
-        Cx<R> psi[12], u[9];
         if constexpr (F) __builtin_amdgcn_sched_barrier(0);
-        staged(tf, psi);
-        U.load(0, s, u);
-        hop_accumulate<0, -1, false>(acc, psi, u, a.twisted[0] != 0, Cx<R>{(R)a.ph_re[0], (R)a.ph_im[0]});
-        staged(tb, psi);
-        U.load(0, neighbour<0, -1>(st, a), u);
-        hop_accumulate<0, +1, true>(acc, psi, u, a.twisted[0] != 0, Cx<R>{(R)a.ph_re[0], -(R)a.ph_im[0]});
+        staged_hop(HopTag<0, -1, false>{}, tf, s, Cx<R>{(R)a.ph_re[0], (R)a.ph_im[0]});
+        staged_hop(HopTag<0, +1, true>{}, tb, neighbour<0, -1>(st, a), Cx<R>{(R)a.ph_re[0], -(R)a.ph_im[0]});
         if constexpr (F) __builtin_amdgcn_sched_barrier(0);
-        staged(xf, psi);
-        U.load(1, s, u);
-        hop_accumulate<1, -1, false>(acc, psi, u, a.twisted[1] != 0, Cx<R>{(R)a.ph_re[1], (R)a.ph_im[1]});
-        staged(xbk, psi);
-        U.load(1, neighbour<1, -1>(st, a), u);
-        hop_accumulate<1, +1, true>(acc, psi, u, a.twisted[1] != 0, Cx<R>{(R)a.ph_re[1], -(R)a.ph_im[1]});
+        staged_hop(HopTag<1, -1, false>{}, xf, s, Cx<R>{(R)a.ph_re[1], (R)a.ph_im[1]});
+        staged_hop(HopTag<1, +1, true>{}, xbk, neighbour<1, -1>(st, a), Cx<R>{(R)a.ph_re[1], -(R)a.ph_im[1]});
       }
       hop_dir<2, PS, PG, R, F>(acc, a, st, in, U);
       if constexpr (F) __builtin_amdgcn_sched_barrier(0);

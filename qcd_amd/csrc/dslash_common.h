@@ -52,11 +52,16 @@ __device__ __forceinline__ T from_cx(Cx<R> v) {
 }
 
 // ------------------------------------------------------------ accessors
-// Component base pointers travel as kernel arguments (scalar registers); a
-// lane addresses every component with one 32-bit byte offset, so loads take
-// the scalar-base + vector-offset form and need no per-lane 64-bit addresses.
-// Host code guarantees 12 * stride * sizeof(complex) < 2^32 (one rhs of a
-// spinor, one direction of a gauge field).
+// Base pointers travel as kernel arguments (scalar registers); a lane
+// addresses every component with one 32-bit byte offset plus the component's
+// distance from its base, which the load encodes as an immediate (at most
+// 4095 bytes on gfx950).  So an accessor keeps one base per group of kPer
+// components (all 12 spinor components of FP16, 8 of FP32, 4 of FP64) and a
+// load needs no address arithmetic beyond the per-site offset; one pointer per
+// component would not fit the scalar register file with four fields and
+// forces spills into vector registers.  Host code guarantees
+// 12 * stride * sizeof(complex) < 2^32 (one rhs of a spinor, one direction
+// of a gauge field).
 template <typename T>
 __device__ __forceinline__ T ld_plane(const T *base, unsigned off) {
   return *(const T *)((const char *)base + off);
@@ -66,18 +71,25 @@ __device__ __forceinline__ void st_plane(T *base, unsigned off, T v) {
   *(T *)((char *)base + off) = v;
 }
 
+// components per base pointer for complex type T, and bases for n components
+template <typename T>
+constexpr int comp_per_base() { return 4096 / (kTile * (int)sizeof(T)); }
+template <typename T, int N>
+constexpr int num_bases() { return (N + comp_per_base<T>() - 1) / comp_per_base<T>(); }
+
 template <int P>
 struct SpinorIn {
   using T = typename Prec<P>::T;
   using R = typename Prec<P>::R;
-  const T *p[12];        // component k of tile 0
+  static constexpr int kPer = comp_per_base<T>(), kNb = num_bases<T, 12>();
+  const T *p[kNb];       // component kPer*i of tile 0
   const float *norm;
   unsigned vstride;      // elements per vector (advance by whole vectors)
   static SpinorIn of(qcd_hip_field f, int rhs) {
     SpinorIn s;
     const size_t st = (size_t)f->lat->stride;
     const T *b = (const T *)f->data + (size_t)rhs * 12 * st;
-    for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
+    for (int i = 0; i < kNb; i++) s.p[i] = b + i * kPer * kTile;
     s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
     s.vstride = (unsigned)(12 * st);
     return s;
@@ -87,10 +99,12 @@ struct SpinorIn {
     SpinorIn s = *this;
     const size_t e = (size_t)nv * vstride;
 #pragma unroll
-    for (int k = 0; k < 12; k++) s.p[k] += e;
+    for (int i = 0; i < kNb; i++) s.p[i] += e;
     if (norm) s.norm += e / 12;
     return s;
   }
+  // component k of tile 0
+  __device__ __forceinline__ const T *comp(int k) const { return p[k / kPer] + (k % kPer) * kTile; }
   // byte offset of site s from its tile-0 component pointer
   static __device__ __forceinline__ unsigned offset(unsigned s) {
     return ((s / kTile) * (12u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
@@ -99,7 +113,7 @@ struct SpinorIn {
   __device__ __forceinline__ void load_raw(unsigned s, T r[12], float &n) const {
     const unsigned off = offset(s);
 #pragma unroll
-    for (int k = 0; k < 12; k++) r[k] = ld_plane(p[k], off);
+    for (int k = 0; k < 12; k++) r[k] = ld_plane(comp(k), off);
     if constexpr (P == QCD_HIP_F16) n = ld_plane(norm, s * 4u);
   }
   static __device__ __forceinline__ void convert(const T r[12], float n, Cx<R> v[12]) {
@@ -125,14 +139,15 @@ template <int P>
 struct SpinorOut {
   using T = typename Prec<P>::T;
   using R = typename Prec<P>::R;
-  T *p[12];
+  static constexpr int kPer = comp_per_base<T>(), kNb = num_bases<T, 12>();
+  T *p[kNb];
   float *norm;
   unsigned vstride;
   static SpinorOut of(qcd_hip_field f, int rhs) {
     SpinorOut s;
     const size_t st = (size_t)f->lat->stride;
     T *b = (T *)f->data + (size_t)rhs * 12 * st;
-    for (int k = 0; k < 12; k++) s.p[k] = b + k * kTile;
+    for (int i = 0; i < kNb; i++) s.p[i] = b + i * kPer * kTile;
     s.norm = f->norm ? f->norm + (size_t)rhs * st : nullptr;
     s.vstride = (unsigned)(12 * st);
     return s;
@@ -141,10 +156,11 @@ struct SpinorOut {
     SpinorOut s = *this;
     const size_t e = (size_t)nv * vstride;
 #pragma unroll
-    for (int k = 0; k < 12; k++) s.p[k] += e;
+    for (int i = 0; i < kNb; i++) s.p[i] += e;
     if (norm) s.norm += e / 12;
     return s;
   }
+  __device__ __forceinline__ T *comp(int k) const { return p[k / kPer] + (k % kPer) * kTile; }
   __device__ __forceinline__ void store(unsigned s, const Cx<R> v[12]) const {
     const unsigned off = SpinorIn<P>::offset(s);
     if constexpr (P == QCD_HIP_F16) {
@@ -157,10 +173,10 @@ struct SpinorOut {
       const R inv = mx > R(0) ? R(1) / mx : R(0);
       st_plane(norm, s * 4u, (float)mx);
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(p[k], off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
+      for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(p[k], off, from_cx<T>(v[k]));
+      for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(v[k]));
     }
   }
 };
@@ -168,19 +184,28 @@ struct SpinorOut {
 template <int P>
 struct GaugeIn {
   using T = typename Prec<P>::T;
-  const T *p[36];        // link component k of direction mu, tile 0, at p[9 mu + k]
+  static constexpr int kPer = comp_per_base<T>(), kNb = num_bases<T, 9>();
+  const T *p[4 * kNb];   // link component kPer*i of direction mu, tile 0, at p[kNb mu + i]
   static GaugeIn of(qcd_hip_field f) {
     GaugeIn g;
     const size_t st = (size_t)f->lat->stride;
     for (int mu = 0; mu < 4; mu++)
-      for (int k = 0; k < 9; k++) g.p[9 * mu + k] = (const T *)f->data + (size_t)mu * 9 * st + k * kTile;
+      for (int i = 0; i < kNb; i++) g.p[kNb * mu + i] = (const T *)f->data + (size_t)mu * 9 * st + i * kPer * kTile;
     return g;
+  }
+  __device__ __forceinline__ const T *comp(int mu, int k) const { return p[kNb * mu + k / kPer] + (k % kPer) * kTile; }
+  // stored values of the link of direction mu at site s
+  __device__ __forceinline__ void load_raw(int mu, unsigned s, T r[9]) const {
+    const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+#pragma unroll
+    for (int k = 0; k < 9; k++) r[k] = ld_plane(comp(mu, k), off);
   }
   template <typename R>
   __device__ __forceinline__ void load(int mu, unsigned s, Cx<R> u[9]) const {
-    const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+    T r[9];
+    load_raw(mu, s, r);
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = to_cx<R>(ld_plane(p[9 * mu + k], off));
+    for (int k = 0; k < 9; k++) u[k] = to_cx<R>(r[k]);
   }
 };
 
@@ -383,37 +408,6 @@ __device__ __forceinline__ Cx<R> diag_term(Cx<R> v, R mp4, R gtm) {
   return Cx<R>{R(-2) * fma(mp4, v.re, -(gtm * v.im)), R(-2) * fma(mp4, v.im, gtm * v.re)};
 }
 
-// accumulate one hop: phase * link * (1 + S g_MU) psi, rebuilt to 4 spins
-template <int MU, int S, bool DAG, typename R>
-__device__ __forceinline__ void hop_accumulate(Cx<R> acc[4][3], const Cx<R> psi[12], const Cx<R> u[9], bool twisted,
-                                               Cx<R> ph) {
-  using PJ = Proj<MU>;
-  constexpr int sp = S > 0 ? 0 : 2;
-  Cx<R> h0[3], h1[3], g0[3], g1[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    h0[c] = cadd(psi[c], ipow<PJ::A + sp>(psi[3 * PJ::P + c]));
-    h1[c] = cadd(psi[3 + c], ipow<PJ::B + sp>(psi[3 * PJ::Q + c]));
-  }
-  su3_mul<DAG>(g0, u, h0);
-  su3_mul<DAG>(g1, u, h1);
-  // Unconditional: an untwisted direction carries the phase (1, 0), which
-  // multiplies exactly and avoids a divergent-looking branch that doubles
-  // the live registers.
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    g0[c] = cmul_ph(g0[c], ph);
-    g1[c] = cmul_ph(g1[c], ph);
-  }
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    acc[0][c] = cadd(acc[0][c], g0[c]);
-    acc[1][c] = cadd(acc[1][c], g1[c]);
-    acc[2][c] = cadd(acc[2][c], ipow<PJ::C + sp>(PJ::R == 0 ? g0[c] : g1[c]));
-    acc[3][c] = cadd(acc[3][c], ipow<PJ::D + sp>(PJ::W == 0 ? g0[c] : g1[c]));
-  }
-}
-
 // the colour-multiplied, phased half spinor g of one hop (before rebuilding)
 template <int MU, int S, bool DAG, typename R>
 __device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> psi[12], const Cx<R> u[9], bool twisted,
@@ -438,6 +432,132 @@ __device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> p
   }
 }
 
+// ------------------------------------------------------------ FP16 hops
+// With FP16 spinors and links a hop works on the stored values: the spin
+// projection is packed FP16 on the unit-normalised components (|h| <= 2), the
+// colour product uses v_dot2_f32_f16 (two FP16 products added to an FP32
+// accumulator), and the site norm is folded into the phase factor that
+// multiplies the six colour products.  About half the vector ALU work of
+// converting the 12 components and 9 links to FP32 first.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ h2v as_h2v(__half2 a) { return __builtin_bit_cast(h2v, a); }
+
+template <int POW>
+__device__ __forceinline__ h2v ipow_h(h2v v) {
+  constexpr int p = POW & 3;
+  if constexpr (p == 0) return v;
+  else if constexpr (p == 1) return h2v{-v.y, v.x};
+  else if constexpr (p == 2) return -v;
+  else return h2v{v.y, -v.x};
+}
+
+// y = U x (DAG = false) or U^+ x (DAG = true) with v_dot2_f32_f16 (two FP16
+// products added to an FP32 accumulator) on (re, im) pairs:
+//   U x:   re = u . (x.re, -x.im),  im = u . (x.im, x.re)
+//   U^+ x: re = u . (x.re,  x.im),  im = u . (x.im, -x.re)
+// The negated high half is the instruction's neg_hi source modifier (the
+// compiler would build such operands with xors); the swapped pair xs takes
+// one instruction per colour (the instruction has no op_sel on gfx950).
+// The 18 products form one asm block so their order is fixed: the six sums
+// advance together, each accumulator read (as the addend, which needs no
+// wait states) six instructions after it was written.  A DOT result read by
+// any other VALU instruction needs 3 wait states on CDNA3/4, which the
+// compiler cannot insert for an asm block, hence the closing s_nop 2.
+template <bool DAG>
+__device__ __forceinline__ void su3_mul_h(Cx<float> y[3], const __half2 u[9], const h2v x[3]) {
+  h2v m[9], xs[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) xs[k] = h2v{x[k].y, x[k].x};
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) m[3 * r + k] = as_h2v(DAG ? u[3 * k + r] : u[3 * r + k]);
+  float o[6];
+#define QH_DOT_OPERANDS                                                                                       \
+  : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(o[4]), "=&v"(o[5])                            \
+  : "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]),       \
+    "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(xs[0]), "v"(xs[1]), "v"(xs[2])
+  if constexpr (DAG) {
+    asm(
+        "v_dot2_f32_f16 %0, %6, %15, 0\n\t"
+        "v_dot2_f32_f16 %1, %6, %18, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %2, %9, %15, 0\n\t"
+        "v_dot2_f32_f16 %3, %9, %18, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %4, %12, %15, 0\n\t"
+        "v_dot2_f32_f16 %5, %12, %18, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %0, %7, %16, %0\n\t"
+        "v_dot2_f32_f16 %1, %7, %19, %1 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %2, %10, %16, %2\n\t"
+        "v_dot2_f32_f16 %3, %10, %19, %3 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %4, %13, %16, %4\n\t"
+        "v_dot2_f32_f16 %5, %13, %19, %5 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %0, %8, %17, %0\n\t"
+        "v_dot2_f32_f16 %1, %8, %20, %1 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %2, %11, %17, %2\n\t"
+        "v_dot2_f32_f16 %3, %11, %20, %3 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %4, %14, %17, %4\n\t"
+        "v_dot2_f32_f16 %5, %14, %20, %5 neg_hi:[0,1,0]\n\t"
+        "s_nop 2"
+        QH_DOT_OPERANDS);
+  } else {
+    asm(
+        "v_dot2_f32_f16 %0, %6, %15, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %1, %6, %18, 0\n\t"
+        "v_dot2_f32_f16 %2, %9, %15, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %3, %9, %18, 0\n\t"
+        "v_dot2_f32_f16 %4, %12, %15, 0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %5, %12, %18, 0\n\t"
+        "v_dot2_f32_f16 %0, %7, %16, %0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %1, %7, %19, %1\n\t"
+        "v_dot2_f32_f16 %2, %10, %16, %2 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %3, %10, %19, %3\n\t"
+        "v_dot2_f32_f16 %4, %13, %16, %4 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %5, %13, %19, %5\n\t"
+        "v_dot2_f32_f16 %0, %8, %17, %0 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %1, %8, %20, %1\n\t"
+        "v_dot2_f32_f16 %2, %11, %17, %2 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %3, %11, %20, %3\n\t"
+        "v_dot2_f32_f16 %4, %14, %17, %4 neg_hi:[0,1,0]\n\t"
+        "v_dot2_f32_f16 %5, %14, %20, %5\n\t"
+        "s_nop 2"
+        QH_DOT_OPERANDS);
+  }
+#undef QH_DOT_OPERANDS
+#pragma unroll
+  for (int r = 0; r < 3; r++) y[r] = Cx<float>{o[2 * r], o[2 * r + 1]};
+}
+
+// hop_half from stored values: spinor components r with site norm n (FP16
+// storage; n is unused otherwise) and link components u
+template <int MU, int S, bool DAG, int PS, int PG, typename R>
+__device__ __forceinline__ void hop_half_raw(Cx<R> g0[3], Cx<R> g1[3], const typename Prec<PS>::T r[12], float n,
+                                             const typename Prec<PG>::T u[9], Cx<R> ph) {
+  if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16) {
+    using PJ = Proj<MU>;
+    constexpr int sp = S > 0 ? 0 : 2;
+    h2v h0[3], h1[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      h0[c] = as_h2v(r[c]) + ipow_h<PJ::A + sp>(as_h2v(r[3 * PJ::P + c]));
+      h1[c] = as_h2v(r[3 + c]) + ipow_h<PJ::B + sp>(as_h2v(r[3 * PJ::Q + c]));
+    }
+    su3_mul_h<DAG>(g0, u, h0);
+    su3_mul_h<DAG>(g1, u, h1);
+    const Cx<R> f{n * ph.re, n * ph.im};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      g0[c] = cmul_ph(g0[c], f);
+      g1[c] = cmul_ph(g1[c], f);
+    }
+  } else {
+    Cx<R> psi[12], uc[9];
+    SpinorIn<PS>::convert(r, n, psi);
+#pragma unroll
+    for (int k = 0; k < 9; k++) uc[k] = to_cx<R>(u[k]);
+    hop_half<MU, S, DAG>(g0, g1, psi, uc, false, ph);
+  }
+}
+
 // add a half spinor of (1 + S g_MU) rebuilt to four spins
 template <int MU, int S, typename R>
 __device__ __forceinline__ void rebuild_add(Cx<R> acc[4][3], const Cx<R> g0[3], const Cx<R> g1[3]) {
@@ -452,6 +572,27 @@ __device__ __forceinline__ void rebuild_add(Cx<R> acc[4][3], const Cx<R> g0[3], 
   }
 }
 
+// hop_half of the spinor at site sn of `in` with the link of direction MU at sl
+template <int MU, int S, bool DAG, int PS, int PG, typename R>
+__device__ __forceinline__ void hop_half_at(Cx<R> g0[3], Cx<R> g1[3], const SpinorIn<PS> &in, unsigned sn,
+                                            const GaugeIn<PG> &U, unsigned sl, Cx<R> ph) {
+  typename Prec<PS>::T r[12];
+  typename Prec<PG>::T u[9];
+  float n = 0.f;
+  in.load_raw(sn, r, n);
+  U.load_raw(MU, sl, u);
+  hop_half_raw<MU, S, DAG, PS, PG>(g0, g1, r, n, u, ph);
+}
+
+// the same hop added to acc (rebuilt to four spins)
+template <int MU, int S, bool DAG, int PS, int PG, typename R>
+__device__ __forceinline__ void hop_acc_at(Cx<R> acc[4][3], const SpinorIn<PS> &in, unsigned sn, const GaugeIn<PG> &U,
+                                           unsigned sl, Cx<R> ph) {
+  Cx<R> g0[3], g1[3];
+  hop_half_at<MU, S, DAG>(g0, g1, in, sn, U, sl, ph);
+  rebuild_add<MU, S>(acc, g0, g1);
+}
+
 // forward hop U_mu(x)(1-g_mu)psi(x+mu) e^{+i th/L} and backward hop
 // U_mu(x-mu)^+ (1+g_mu) psi(x-mu) e^{-i th/L}
 // FENCE: keep the loads of this direction from being hoisted above the
@@ -460,22 +601,11 @@ template <int MU, int PS, int PG, typename R, bool FENCE = false>
 __device__ __forceinline__ void hop_dir(Cx<R> acc[4][3], const DslashArgs &a, const Site &st, const SpinorIn<PS> &in,
                                         const GaugeIn<PG> &U) {
   if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
-  const bool tw = a.twisted[MU] != 0;
   const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
   const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
-  Cx<R> psi[12], u[9];
-  {
-    const unsigned sf = neighbour<MU, +1>(st, a);
-    in.load(sf, psi);
-    U.load(MU, st.s, u);
-    hop_accumulate<MU, -1, false>(acc, psi, u, tw, php);
-  }
-  {
-    const unsigned sb = neighbour<MU, -1>(st, a);
-    in.load(sb, psi);
-    U.load(MU, sb, u);
-    hop_accumulate<MU, +1, true>(acc, psi, u, tw, phm);
-  }
+  hop_acc_at<MU, -1, false>(acc, in, neighbour<MU, +1>(st, a), U, st.s, php);
+  const unsigned sb = neighbour<MU, -1>(st, a);
+  hop_acc_at<MU, +1, true>(acc, in, sb, U, sb, phm);
 }
 
 // ------------------------------------------------------------ epilogues

@@ -58,22 +58,11 @@ __device__ __forceinline__ void eo_mass(Cx<R> v[12], const EoArgs &e) {
 template <int MU, int PS, int PG, typename R>
 __device__ __forceinline__ void eo_hop_dir(Cx<R> acc[4][3], const DslashArgs &a, const Site &st,
                                            const SpinorIn<PS> &in, const GaugeIn<PG> &U) {
-  const bool tw = a.twisted[MU] != 0;
   const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
   const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
-  Cx<R> psi[12], u[9];
-  {
-    const unsigned sf = neighbour<MU, +1>(st, a);
-    in.load(sf >> 1, psi);
-    U.load(MU, st.s, u);
-    hop_accumulate<MU, -1, false>(acc, psi, u, tw, php);
-  }
-  {
-    const unsigned sb = neighbour<MU, -1>(st, a);
-    in.load(sb >> 1, psi);
-    U.load(MU, sb, u);
-    hop_accumulate<MU, +1, true>(acc, psi, u, tw, phm);
-  }
+  hop_acc_at<MU, -1, false>(acc, in, neighbour<MU, +1>(st, a) >> 1, U, st.s, php);
+  const unsigned sb = neighbour<MU, -1>(st, a);
+  hop_acc_at<MU, +1, true>(acc, in, sb >> 1, U, sb, phm);
 }
 
 // one lane per output site; grid.y runs over the batch rows (DslashArgs::ids)
