@@ -137,6 +137,10 @@ typedef struct {
                           the Schur complement D^ = M - H_oe M^-1 H_eo on odd
                           sites (half-length vectors) and the outer loop the
                           full D x = b; needs every extent even; op is then D */
+   int    graph;       /* 1 (default): each GCR cycle is captured once as a
+                          HIP graph and replayed (one launch instead of
+                          about 7 per GCR step); 0: launched kernel by kernel.
+                          Both run the same kernels in the same order. */
 } qcd_hip_solver_params;
 typedef struct {
    int    outer;        /* outer cycles (summed over rhs) */

@@ -67,7 +67,7 @@ class SolverParams(ctypes.Structure):
     """qcd_hip_solver_params (include/qcd_hip.h)."""
     _fields_ = [("n_restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("tol", ctypes.c_double),
                 ("inner_prec", ctypes.c_int), ("op", ctypes.c_int), ("keep_q1", ctypes.c_int),
-                ("batch", ctypes.c_int), ("eo", ctypes.c_int)]
+                ("batch", ctypes.c_int), ("eo", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

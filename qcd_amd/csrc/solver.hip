@@ -416,6 +416,17 @@ struct qcd_hip_solver_s {
   unsigned Vi = 0, nbi = 0;          // inner vector sites and site-kernel workgroups
   qcd_hip_field te = nullptr;        // EO: B inner even temporaries of the matvec
   qcd_hip_field re64 = nullptr, ro64 = nullptr, t64 = nullptr, eo64 = nullptr;   // EO: B FP64 half vectors each
+  // the captured GCR cycle (prm.graph) and the launch arguments it was
+  // captured with; every buffer it touches belongs to the solver
+  hipGraphExec_t cycle = nullptr;
+  int cycle_count = 0;
+  double cycle_kappa = 0.0, cycle_mu = 0.0;
+  void drop_cycle() {
+    if (!cycle) return;
+    (void)hipStreamSynchronize(stream);   // no replay may still be running
+    (void)hipGraphExecDestroy(cycle);
+    cycle = nullptr;
+  }
 };
 
 namespace {
@@ -453,6 +464,37 @@ int gcr_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream
   return 0;
 }
 
+// One GCR cycle on stream st.  With prm.graph the launches of gcr_cycle are
+// captured into a graph once per (active rhs count, kappa, mu) and the graph
+// is replayed: the kernels read the active-slot list and every coefficient
+// from device memory, so a replay is the same work as the eager launches.
+// The cycle records no memset or copy nodes.
+template <int P>
+int run_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream_t st) {
+  if (!S->prm.graph) return gcr_cycle<P>(S, count, kappa, mu, st);
+  if (!S->cycle || S->cycle_count != count || S->cycle_kappa != kappa || S->cycle_mu != mu) {
+    S->drop_cycle();
+    QH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = gcr_cycle<P>(S, count, kappa, mu, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    QH_CHECK(e);
+    const hipError_t ei = hipGraphInstantiate(&S->cycle, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) S->cycle = nullptr;
+    QH_CHECK(ei);
+    S->cycle_count = count;
+    S->cycle_kappa = kappa;
+    S->cycle_mu = mu;
+  }
+  QH_CHECK(hipGraphLaunch(S->cycle, st));
+  return 0;
+}
+
 template <int P>
 int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
   const dim3 g2(S->nbi, (unsigned)count);
@@ -460,7 +502,7 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
   if (!S->prm.eo) {
     k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc, SpinorSet<P>::of(S->Rs),
                                             S->prm.n_restart, S->ids, V);
-    QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
+    QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
     k_accumulate<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(x), x0, SpinorSet<P>::of(S->E), S->sc, S->ids,
                                            V);
     QH_CHECK(hipGetLastError());
@@ -477,7 +519,7 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
   k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->eo64), S->sc, SpinorSet<P>::of(S->Rs),
                                           S->prm.n_restart, S->ids, V);
   QH_CHECK(hipGetLastError());
-  QH_TRY(gcr_cycle<P>(S, count, kappa, mu, st));
+  QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
   QH_TRY(eo_scale64(S->lat, S->E, S->eo64, &S->sc[0].rnorm, sizeof(GcrScal) / sizeof(double), S->ids, count, st));
   QH_TRY(eo_launch(EO_SUB_MINV, false, S->t64, 0, 1, S->eo64, 0, 1, S->g64, S->re64, 0, 1, S->ids, count, m, mu, 0,
                    nullptr, st, nullptr));
@@ -569,6 +611,7 @@ void qcd_hip_solver_default_params(qcd_hip_solver_params *p) {
   p->keep_q1 = 0;
   p->batch = 1;
   p->eo = 0;
+  p->graph = 1;
 }
 
 int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hip_solver_params *prm) {
@@ -576,8 +619,10 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   QH_TRY(ensure_init());
   if (!lat || !prm || prm->n_restart < 1 || prm->n_restart > kN || prm->tol <= 0.0 || prm->max_outer < 1 ||
       prm->inner_prec < QCD_HIP_F64 || prm->inner_prec > QCD_HIP_F16 || prm->batch < 1 || prm->batch > 1024 ||
-      (prm->op != QCD_HIP_OP_Q && prm->op != QCD_HIP_OP_D) || (prm->eo != 0 && prm->eo != 1)) {
-    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1, batch 1..1024, eo 0/1)", kN);
+      (prm->op != QCD_HIP_OP_Q && prm->op != QCD_HIP_OP_D) || (prm->eo != 0 && prm->eo != 1) ||
+      (prm->graph != 0 && prm->graph != 1)) {
+    set_error("solver_create: bad parameters (n_restart 1..%d, tol > 0, max_outer >= 1, batch 1..1024, eo 0/1, "
+              "graph 0/1)", kN);
     return 1;
   }
   if (prm->eo && !eo_supported(lat)) {
@@ -632,6 +677,7 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
 
 void qcd_hip_solver_destroy(qcd_hip_solver S) {
   if (!S) return;
+  S->drop_cycle();
   qcd_hip_field_destroy(S->Rs);
   qcd_hip_field_destroy(S->Ws);
   qcd_hip_field_destroy(S->E);
@@ -660,6 +706,7 @@ int qcd_hip_solver_set_gauge(qcd_hip_solver S, qcd_hip_field g64) {
     return 1;
   }
   S->g64 = g64;
+  S->drop_cycle();   // the captured cycle holds the previous field's pointers
   if (S->prm.inner_prec == QCD_HIP_F64) {
     if (S->gin && S->gin != g64) qcd_hip_field_destroy(S->gin);
     S->gin = g64;

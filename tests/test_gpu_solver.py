@@ -127,3 +127,39 @@ def test_scratch_fields_outlive_destroyed_lattices(gpu):
         assert abs(res - ref) <= 1e-12 * ref
         del g, fx, fb, lat
         gc.collect()
+
+
+@pytest.mark.parametrize("prec,eo", [("f64", 0), ("f32", 1), ("f16", 1)])
+def test_graph_replayed_cycles_match_eager(setup, prec, eo):
+    """GCR cycles replayed from a captured HIP graph give the same bits as the
+    kernel-by-kernel launches: over many replays per solve (with the outer
+    FP64 residual, host syncs and slot-list uploads between them), after the
+    active rhs count shrinks (re-capture), on a second solve that reuses the
+    graph, at a new kappa, and after the gauge field is set again."""
+    Q, u, lat, g, b, _ = setup
+    fb = Q.Spinor(lat, nrhs=4)
+    for r, (site, sp, c) in enumerate([((0, 0, 0, 0), 1, 2), ((3, 1, 2, 5), 0, 0),
+                                       ((7, 7, 1, 0), 2, 1), ((15, 4, 6, 3), 3, 2)]):
+        fb.upload(Q.point_source(L, site, sp, c), rhs=r)
+    runs = {}
+    for graph in (0, 1):
+        S = Q.Solver(lat, g, n_restart=8, tol=1e-10, inner_prec=prec, eo=eo, batch=4, graph=graph,
+                     max_outer=2000)
+        out = []
+        for kappa, mu in ((0.13, 0.0), (0.13, 0.0), (0.128, 0.03)):
+            fx = Q.Spinor(lat, nrhs=4)
+            st = S.solve(fx, fb, kappa, mu)
+            assert st["relres"] <= 1e-10 and st["outer"] >= 8
+            out.append([fx.download(r) for r in range(4)])
+        S.set_gauge(g)
+        fx = Q.Spinor(lat, nrhs=4)
+        S.solve(fx, fb, 0.13)
+        out.append([fx.download(r) for r in range(4)])
+        S.close()
+        runs[graph] = out
+    for a, e in zip(runs[1], runs[0]):
+        for r in range(4):
+            assert np.array_equal(a[r], e[r])
+    for r in range(4):   # the same solve twice, and after set_gauge, is the same
+        assert np.array_equal(runs[1][0][r], runs[1][1][r])
+        assert np.array_equal(runs[1][0][r], runs[1][3][r])
