@@ -20,6 +20,7 @@
 #include <mutex>
 #include <cfloat>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
