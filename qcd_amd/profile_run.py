@@ -4,6 +4,11 @@ usage: python -m qcd_amd.profile_run --lattice 48 24 24 24 --prec f64 --iters 20
 Runs `iters` Wilson Dslash applications and `iters` plain FP64 copies
 (qcd_hip_lincomb with a = 1, b = 0: 192 B read + 192 B written per site) on
 the same lattice, so that the counters of the copy calibrate those of Dslash.
+
+       python -m qcd_amd.profile_run --solve --lattice 48 24 24 24 --prec f16 --rhs 48 --eo 1
+Runs one propagator solve instead: 12 point sources (spin x colour) at each
+of rhs/12 sites, kappa 0.132, eps 0.3, GCR(32), tol 1e-10, all rhs in one
+batch, so that a kernel trace shows where a solve spends its time.
 """
 import argparse
 import os
@@ -24,9 +29,14 @@ def main(argv=None):
                     help="force a traversal (0 XCD ranges, 1 y-slabs, 2 + 256*n z-march of n steps)")
     ap.add_argument("--occ", type=int, default=0, help="occupancy variant when --map is given")
     ap.add_argument("--sweep", action="store_true", help="time every candidate (QCD_HIP_TUNE_VERBOSE)")
+    ap.add_argument("--solve", action="store_true", help="profile one batched propagator solve instead")
+    ap.add_argument("--rhs", type=int, default=12, help="--solve: right-hand sides (a multiple of 12)")
+    ap.add_argument("--eo", type=int, default=1, help="--solve: even-odd preconditioning")
     args = ap.parse_args(argv)
     L = tuple(args.lattice)
     Q.init(0)
+    if args.solve:
+        return solve(Q, L, args.prec, args.rhs, args.eo)
     lat = Q.Lattice(L)
     u = Q.gauge_generate(L, 0.3, 4001)
     psi = Q.spinor_gaussian(L, 4002)
@@ -53,6 +63,25 @@ def main(argv=None):
     V = int(np.prod(L))
     print(f"lattice {L} V={V} prec={args.prec} dslash mean {ms*1e3:.2f} us; copy bytes/launch {V*384}",
           flush=True)
+    return 0
+
+
+def solve(Q, L, prec, nrhs, eo):
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat)
+    Q.gauge_generate_device(g, 0.3, 3003)
+    rng = np.random.default_rng(3003)
+    sites = [tuple(int(rng.integers(0, L[d])) for d in range(4)) for _ in range(max(1, nrhs // 12))]
+    b = Q.Spinor(lat, nrhs=12 * len(sites))
+    for i, site in enumerate(sites):
+        for r in range(12):
+            Q.point_source_device(b, 12 * i + r, site, r // 3, r % 3)
+    x = Q.Spinor(lat, nrhs=12 * len(sites))
+    S = Q.Solver(lat, g, n_restart=32, tol=1e-10, inner_prec=prec, batch=12 * len(sites), eo=eo, max_outer=20000)
+    st = S.solve(x, b, 0.132)
+    Q.synchronize()
+    print(f"lattice {L} prec={prec} rhs={12 * len(sites)} eo={eo}: {st['seconds']:.3f} s, "
+          f"{st['inner_iters']} GCR iterations, {st['outer']} outer cycles, relres {st['relres']:.2e}", flush=True)
     return 0
 
 
