@@ -171,6 +171,12 @@ int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hi
  * copies; alpha = 0 projects the spatial links (qcd_blas.c:940-1364). */
 int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nsteps);
 
+/* nsteps of stout smearing (lib/src/qcd_smearing.c:374-476) of the FP64
+ * gauge field in into out (distinct fields on one lattice), ndim 3 (spatial,
+ * temporal links copied) or 4; nsteps = 0 copies.  exp(iQ) by the
+ * Peardon-Morningstar formula (qcd_smearing.c:293-370), exact at Q = 0. */
+int qcd_hip_stout_smear(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, int nsteps);
+
 /* mean plaquette of an FP64 gauge field (lib/src/qcd_observables.c:15-36) */
 int qcd_hip_plaquette(qcd_hip_field gauge64, double *plaq);
 

@@ -25,6 +25,13 @@ int qcd_gaussIteration3dAll(qcd_vector *v, qcd_gaugeField *u, qcd_real_8 alpha, 
  * spatial links -> P_SU3[ U + alpha * sum of the 4 spatial staples ], temporal
  * links copied.  Runs on the GPU; apeu may alias u. */
 int qcd_apeSmear3d(qcd_gaugeField *apeu, qcd_gaugeField *u, qcd_real_8 alpha);
+/* One stout smearing step (include/qcd_smearing.h:42, lib/src/qcd_smearing.c:374-476):
+ * U' = exp(iQ) U with Q the traceless Hermitian part of rho * (staples) U^+,
+ * ndim = 3 (spatial links, temporal copied) or 4.  Runs on the GPU; u_out
+ * may alias u.  The reference exits the process on another ndim; this
+ * returns 1 with a message.  Where the reference's exponential divides 0 by
+ * 0 (Q = 0, e.g. the unit gauge field) the result here is exact. */
+int qcd_stoutsmearing(qcd_gaugeField *u_out, qcd_gaugeField *u, qcd_real_8 rho, int ndim);
 
 #ifdef __cplusplus
 }

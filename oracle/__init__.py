@@ -64,6 +64,8 @@ def lib():
             "orc_plaquette": (d, [vp, _dp]),
             "orc_project_su3": (None, [_dp]),
             "orc_ape3d": (None, [vp, _dp, _dp, d]),
+            "orc_stout_exp": (None, [_dp]),
+            "orc_stout": (i, [vp, _dp, _dp, d, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -201,6 +203,24 @@ def project_su3(m):
         lib().orc_project_su3(_p(blk))
         a[k] = blk
     return a.reshape(np.shape(m))
+
+
+def stout_exp(q):
+    """exp(iQ) of Hermitian traceless 3x3 matrices q (..., 3, 3) (lib/src/qcd_smearing.c:293-370)."""
+    a = np.array(q, dtype=np.complex128, order="C").reshape(-1, 3, 3)
+    for k in range(a.shape[0]):
+        blk = np.ascontiguousarray(a[k])
+        lib().orc_stout_exp(_p(blk))
+        a[k] = blk
+    return a.reshape(np.shape(q))
+
+
+def stout(geo, u, rho, ndim=3):
+    """One stout smearing step (lib/src/qcd_smearing.c:374-476); returns the new field."""
+    out = np.empty_like(u)
+    if lib().orc_stout(geo._h, _p(out), _p(np.ascontiguousarray(u)), float(rho), int(ndim)):
+        raise ValueError("ndim should be either 3 or 4")
+    return out
 
 
 def ape3d(geo, u, alpha):

@@ -841,3 +841,159 @@ void orc_ape3d(const orc_geo *g, double *out, const double *u, double alpha)
       }
    }
 }
+
+/* ---------------------------------------------------------------- stout */
+/* sin(w)/w with the reference's series below |w| = 0.05 (qcd_smearing.c:279-290) */
+static double stout_xi0(double w)
+{
+   if (fabs(w) < 0.05) {
+      double t = 1.0 - w * w / 42.0;
+      double t1 = 1.0 - w * w / 20.0 * t;
+      return 1.0 - w * w / 6.0 * t1;
+   }
+   return sin(w) / w;
+}
+
+/* exp(iQ) for Hermitian traceless Q in place: Cayley-Hamilton coefficients of
+ * Peardon & Morningstar (hep-lat/0311018 eqs. 19, 29-34) as in
+ * qcd_smearing.c:293-370.  Two differences, both where the reference returns
+ * NaN: for tr(Q^2)/2 < 1e-24 it divides 0 by 0 (acos(NaN)); here exp(iQ) =
+ * 1 + iQ - Q^2/2, exact to double precision at that size.  And for a
+ * degenerate spectrum (two equal eigenvalues, theta = 0) rounding can put
+ * c0 / c0max above 1; here the argument of acos is clamped to 1. */
+void orc_stout_exp(double *q)
+{
+   cplx (*M)[3] = (cplx (*)[3])q;
+   cplx M2[3][3], M3[3][3];
+   int i, j;
+   double c0, c1, c0max, theta, u, w, xi, fac;
+   cplx e2iu, emiu, h0, h1, h2, f0, f1, f2, t, t1;
+   int neg = 0;
+   mat_mul3(M2, M, M);
+   mat_mul3(M3, M2, M);
+   c0 = (M3[0][0].re + M3[1][1].re + M3[2][2].re) / 3.0;
+   c1 = (M2[0][0].re + M2[1][1].re + M2[2][2].re) / 2.0;
+   if (c1 < 1e-24) {
+      for (i = 0; i < 3; i++)
+         for (j = 0; j < 3; j++) {
+            cplx r;
+            r.re = (i == j ? 1.0 : 0.0) - M[i][j].im - 0.5 * M2[i][j].re;
+            r.im = M[i][j].re - 0.5 * M2[i][j].im;
+            M[i][j] = r;
+         }
+      return;
+   }
+   if (c0 < 0) { neg = 1; c0 = -c0; }
+   c0max = 2.0 * pow(c1 / 3.0, 1.5);
+   theta = acos(c0 < c0max ? c0 / c0max : 1.0);   /* the reference: acos(c0 / c0max), NaN when rounding puts it above 1 */
+   u = sqrt(c1 / 3.0) * cos(theta / 3.0);
+   w = sqrt(c1) * sin(theta / 3.0);
+   xi = stout_xi0(w);
+   e2iu.re = cos(2 * u); e2iu.im = sin(2 * u);
+   emiu.re = cos(u); emiu.im = -sin(u);
+   /* h0 = (u^2 - w^2) e^{2iu} + e^{-iu} (8 u^2 cos w + 2iu (3u^2 + w^2) xi0)   eq. (30) */
+   t.re = u * u - w * w; t.im = 0.0;
+   t1.re = 8 * u * u * cos(w); t1.im = 2 * u * (3 * u * u + w * w) * xi;
+   h0 = cadd(cmul(t, e2iu), cmul(emiu, t1));
+   /* h1 = 2u e^{2iu} - e^{-iu} (2u cos w - i (3u^2 - w^2) xi0)   eq. (31) */
+   t.re = 2 * u; t.im = 0.0;
+   t1.re = 2 * u * cos(w); t1.im = -(3 * u * u - w * w) * xi;
+   {
+      cplx a = cmul(t, e2iu), b = cmul(emiu, t1);
+      h1.re = a.re - b.re; h1.im = a.im - b.im;
+   }
+   /* h2 = e^{2iu} - e^{-iu} (cos w + 3iu xi0)   eq. (32) */
+   t.re = cos(w); t.im = 3 * u * xi;
+   {
+      cplx b = cmul(emiu, t);
+      h2.re = e2iu.re - b.re; h2.im = e2iu.im - b.im;
+   }
+   fac = 9 * u * u - w * w;   /* eq. (29) */
+   if (!neg) {
+      f0.re = h0.re / fac; f0.im = h0.im / fac;
+      f1.re = h1.re / fac; f1.im = h1.im / fac;
+      f2.re = h2.re / fac; f2.im = h2.im / fac;
+   } else {   /* eq. (34): f_j(-c0) = (-1)^j conj(f_j(c0)) */
+      f0.re = h0.re / fac; f0.im = -h0.im / fac;
+      f1.re = -h1.re / fac; f1.im = h1.im / fac;
+      f2.re = h2.re / fac; f2.im = -h2.im / fac;
+   }
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) {
+         cplx r = cadd(cmul(f1, M[i][j]), cmul(f2, M2[i][j]));
+         if (i == j) r = cadd(r, f0);
+         M[i][j] = r;
+      }
+}
+
+/* One stout step (qcd_smearing.c:374-476): for the smeared directions mu
+ * (1..3 when ndim = 3, 0..3 when ndim = 4; with ndim = 3 the temporal links
+ * are copied), C = sum over nu != mu of the same range of the backward staple
+ * U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) then the forward staple
+ * U_nu(x) U_mu(x+nu) U_nu(x+mu)^+; W = C U_mu(x)^+; A = W - W^+ minus tr(A)/3;
+ * Q = -i rho/2 A (Hermitian, traceless); U'_mu(x) = exp(iQ) U_mu(x).
+ * out must not alias u; returns 1 for ndim other than 3 or 4. */
+int orc_stout(const orc_geo *g, double *out, const double *u, double rho, int ndim)
+{
+   long l;
+   int start;
+   if (ndim != 3 && ndim != 4) return 1;
+   start = ndim == 3 ? 1 : 0;
+#pragma omp parallel for schedule(static)
+   for (l = 0; l < g->V; l++) {
+      int mu, nu, r, c;
+      memcpy(out + 72 * l, u + 72 * l, 18 * sizeof(double));
+      for (mu = start; mu < 4; mu++) {
+         cplx S[3][3], T[3][3], P[3][3], A[3][3], W[3][3], Q[3][3], E[3][3];
+         const cplx (*U0)[3] = (const cplx (*)[3])(u + 72 * l + 18 * mu);
+         cplx tr;
+         memset(S, 0, sizeof(S));
+         for (nu = start; nu < 4; nu++) {
+            if (nu == mu) continue;
+            {  /* backward: U_nu(x-nu)^+ [U_mu(x-nu) U_nu(x-nu+mu)] */
+               const long lm = g->minus[4 * l + nu];
+               const cplx (*Un)[3] = (const cplx (*)[3])(u + 72 * lm + 18 * nu);
+               const cplx (*Um)[3] = (const cplx (*)[3])(u + 72 * lm + 18 * mu);
+               const cplx (*Ua)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * lm + mu] + 18 * nu);
+               for (r = 0; r < 3; r++)
+                  for (c = 0; c < 3; c++) A[r][c] = conjc(Un[c][r]);
+               mat_mul3(T, Um, Ua);
+               mat_mul3(P, A, T);
+               for (r = 0; r < 3; r++)
+                  for (c = 0; c < 3; c++) S[r][c] = cadd(S[r][c], P[r][c]);
+            }
+            {  /* forward: [U_nu(x) U_mu(x+nu)] U_nu(x+mu)^+ */
+               const cplx (*Un)[3] = (const cplx (*)[3])(u + 72 * l + 18 * nu);
+               const cplx (*Um)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * l + nu] + 18 * mu);
+               const cplx (*Ua)[3] = (const cplx (*)[3])(u + 72 * g->plus[4 * l + mu] + 18 * nu);
+               for (r = 0; r < 3; r++)
+                  for (c = 0; c < 3; c++) A[r][c] = conjc(Ua[c][r]);
+               mat_mul3(T, Un, Um);
+               mat_mul3(P, T, A);
+               for (r = 0; r < 3; r++)
+                  for (c = 0; c < 3; c++) S[r][c] = cadd(S[r][c], P[r][c]);
+            }
+         }
+         for (r = 0; r < 3; r++)
+            for (c = 0; c < 3; c++) A[r][c] = conjc(U0[c][r]);
+         mat_mul3(W, S, A);
+         for (r = 0; r < 3; r++)
+            for (c = 0; c < 3; c++) {
+               P[r][c].re = W[r][c].re - W[c][r].re;
+               P[r][c].im = W[r][c].im + W[c][r].im;
+            }
+         tr.re = (P[0][0].re + P[1][1].re + P[2][2].re) / 3.0;
+         tr.im = (P[0][0].im + P[1][1].im + P[2][2].im) / 3.0;
+         for (r = 0; r < 3; r++) { P[r][r].re -= tr.re; P[r][r].im -= tr.im; }
+         for (r = 0; r < 3; r++)   /* Q = -i (rho/2) P */
+            for (c = 0; c < 3; c++) {
+               Q[r][c].re = 0.5 * rho * P[r][c].im;
+               Q[r][c].im = -0.5 * rho * P[r][c].re;
+            }
+         orc_stout_exp((double *)Q);
+         mat_mul3(E, Q, U0);
+         memcpy(out + 72 * l + 18 * mu, E, sizeof(E));
+      }
+   }
+   return 0;
+}

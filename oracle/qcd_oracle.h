@@ -77,6 +77,13 @@ void orc_project_su3(double *m);
 /* One step of 3d APE smearing of every spatial link, temporal links copied
  * (lib/src/qcd_smearing.c:205-273); out must not alias u. */
 void orc_ape3d(const orc_geo *g, double *out, const double *u, double alpha);
+/* exp(iQ) of one Hermitian traceless 3x3 matrix in place, Peardon-Morningstar
+ * (lib/src/qcd_smearing.c:293-370); 1 + iQ - Q^2/2 when tr(Q^2)/2 < 1e-24,
+ * and acos's argument clamped to 1 (both cases where the reference gives NaN). */
+void orc_stout_exp(double *q);
+/* One stout smearing step, ndim = 3 (spatial, temporal links copied) or 4
+ * (lib/src/qcd_smearing.c:374-476); out must not alias u; 1 on bad ndim. */
+int orc_stout(const orc_geo *g, double *out, const double *u, double rho, int ndim);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,7 @@ def lib():
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
         "qcd_hip_dslash_schur": (i, [_vp, _vp, _vp, d, d]),
         "qcd_hip_ape_smear": (i, [_vp, _vp, d, i]),
+        "qcd_hip_stout_smear": (i, [_vp, _vp, d, i, i]),
         "qcd_hip_plaquette": (i, [_vp, _dp]),
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
         "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
@@ -161,6 +162,7 @@ def lib():
         "qcd_hip_unbindGaugeField": (None, []),
         "qcd_gaussIteration3d": (i, [pf, pf, d, ctypes.c_uint32]),
         "qcd_apeSmear3d": (i, [pf, pf, d]),
+        "qcd_stoutsmearing": (i, [pf, pf, d, i]),
         "qcd_projectSU33d": (None, [pf]),
         "qcd_gaussIteration3dAll": (i, [pf, pf, d, ctypes.c_uint16]),
         "qcd_zeroVector": (None, [pf]),
@@ -347,6 +349,11 @@ def dslash_schur(out, inp, gauge, m, tm=0.0):
 def ape_smear(out, inp, alpha, nsteps=1):
     """nsteps of 3d APE smearing of the FP64 gauge field inp into out (lib/src/qcd_smearing.c:205-273)."""
     _check(lib().qcd_hip_ape_smear(out.h, inp.h, float(alpha), int(nsteps)), "qcd_hip_ape_smear")
+
+
+def stout_smear(out, inp, rho, ndim=3, nsteps=1):
+    """nsteps of stout smearing of the FP64 gauge field inp into out (lib/src/qcd_smearing.c:374-476)."""
+    _check(lib().qcd_hip_stout_smear(out.h, inp.h, float(rho), int(ndim), int(nsteps)), "qcd_hip_stout_smear")
 
 
 def eo_split(a, L):

@@ -1,4 +1,5 @@
-// ape.hip -- 3D APE smearing of the gauge field with SU(3) projection.
+// ape.hip -- 3D APE smearing of the gauge field with SU(3) projection, stout
+// smearing (3d or 4d), and the plaquette.
 //
 // One step (lib/src/qcd_smearing.c:205-273), for the spatial directions
 // mu = 1, 2, 3 (temporal links are copied unchanged):
@@ -172,6 +173,156 @@ __global__ void __launch_bounds__(kBlock) k_plaquette(DslashArgs a, GaugeIn<QCD_
   block_reduce_store(sum, partials, blockIdx.x);
 }
 
+// ------------------------------------------------------------ stout
+// sin(w)/w, with the reference's series below |w| = 0.05 (qcd_smearing.c:279-290)
+__device__ __forceinline__ double stout_xi0(double w) {
+  if (fabs(w) < 0.05) {
+    const double t = 1.0 - w * w / 42.0;
+    const double t1 = 1.0 - w * w / 20.0 * t;
+    return 1.0 - w * w / 6.0 * t1;
+  }
+  return sin(w) / w;
+}
+
+// m <- exp(i m) for Hermitian traceless m: f0 + f1 Q + f2 Q^2 with the
+// Cayley-Hamilton coefficients of Peardon & Morningstar (hep-lat/0311018
+// eqs. 19, 29-34; qcd_smearing.c:293-370).  Where the reference divides 0 by
+// 0 (tr Q^2 / 2 below 1e-24, e.g. the unit gauge field) this uses
+// 1 + iQ - Q^2/2, exact to double precision there; where rounding puts the
+// argument of acos above 1 (two equal eigenvalues) it is clamped to 1.  The
+// reference returns NaN in both cases.
+__device__ void stout_exp(C m[9]) {
+  C m2[9], m3[9];
+  mm3<false, false>(m2, m, m);
+  mm3<false, false>(m3, m2, m);
+  double c0 = (m3[0].re + m3[4].re + m3[8].re) / 3.0;
+  const double c1 = (m2[0].re + m2[4].re + m2[8].re) / 2.0;
+  if (c1 < 1e-24) {
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      m[k] = C{(k % 4 == 0 ? 1.0 : 0.0) - m[k].im - 0.5 * m2[k].re, m[k].re - 0.5 * m2[k].im};
+    return;
+  }
+  const bool neg = c0 < 0.0;
+  if (neg) c0 = -c0;
+  const double c0max = 2.0 * pow(c1 / 3.0, 1.5);
+  const double theta = acos(c0 < c0max ? c0 / c0max : 1.0);   // degenerate spectra: rounding can exceed 1
+  const double u = sqrt(c1 / 3.0) * cos(theta / 3.0);
+  const double w = sqrt(c1) * sin(theta / 3.0);
+  const double xi = stout_xi0(w), cw = cos(w);
+  const C e2iu{cos(2 * u), sin(2 * u)}, emiu{cos(u), -sin(u)};
+  const C h0 = [&] {
+    const C a = cm(C{u * u - w * w, 0.0}, e2iu), b = cm(emiu, C{8 * u * u * cw, 2 * u * (3 * u * u + w * w) * xi});
+    return C{a.re + b.re, a.im + b.im};
+  }();
+  const C h1 = [&] {
+    const C a = cm(C{2 * u, 0.0}, e2iu), b = cm(emiu, C{2 * u * cw, -(3 * u * u - w * w) * xi});
+    return C{a.re - b.re, a.im - b.im};
+  }();
+  const C h2 = [&] {
+    const C b = cm(emiu, C{cw, 3 * u * xi});
+    return C{e2iu.re - b.re, e2iu.im - b.im};
+  }();
+  const double fac = 9 * u * u - w * w;
+  // eq. (34): f_j(-c0) = (-1)^j conj(f_j(c0))
+  const C f0 = neg ? C{h0.re / fac, -h0.im / fac} : C{h0.re / fac, h0.im / fac};
+  const C f1 = neg ? C{-h1.re / fac, h1.im / fac} : C{h1.re / fac, h1.im / fac};
+  const C f2 = neg ? C{h2.re / fac, -h2.im / fac} : C{h2.re / fac, h2.im / fac};
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const C a = cm(f1, m[k]), b = cm(f2, m2[k]);
+    C r{a.re + b.re, a.im + b.im};
+    if (k % 4 == 0) r = C{r.re + f0.re, r.im + f0.im};
+    m[k] = r;
+  }
+}
+
+template <int MU>
+__device__ __forceinline__ unsigned fwd_of(const Site &st, const DslashArgs &a) {
+  return neighbour<MU, +1>(st, a);
+}
+__device__ __forceinline__ unsigned fwd_dir(unsigned mu, const Site &st, const DslashArgs &a) {
+  return mu == 0 ? fwd_of<0>(st, a) : mu == 1 ? fwd_of<1>(st, a) : mu == 2 ? fwd_of<2>(st, a) : fwd_of<3>(st, a);
+}
+__device__ __forceinline__ unsigned bwd_dir(unsigned mu, const Site &st, const DslashArgs &a) {
+  return mu == 0 ? neighbour<0, -1>(st, a) : mu == 1 ? neighbour<1, -1>(st, a)
+                                           : mu == 2 ? neighbour<2, -1>(st, a) : neighbour<3, -1>(st, a);
+}
+
+// One stout step (qcd_smearing.c:374-476), one lane per (site, direction):
+// C = sum over nu != mu (nu from START) of the backward staple then the
+// forward one, W = C U^+, A = W - W^+ minus tr(A)/3, Q = -i rho/2 A,
+// U' = exp(iQ) U.  Directions below START (the temporal one for 3d stout)
+// are copied.
+template <unsigned START>
+__global__ void __launch_bounds__(kBlock) k_stout(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ out,
+                                                  size_t stride, double rho) {
+  const unsigned j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (j >= 4 * a.V) return;
+  const unsigned s = j / 4, mu = j % 4;
+  const Site st = site_coords(s, a);
+  C m[9];
+  U.load(mu, s, m);
+  if (mu >= START) {
+    C S[9], A[9], B[9], T[9], W[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
+    const unsigned sf = fwd_dir(mu, st, a);
+    for (unsigned nu = START; nu < 4; nu++) {
+      if (nu == mu) continue;
+      // backward: U_nu(x-nu)^+ [U_mu(x-nu) U_nu(x-nu+mu)]
+      const unsigned sb = bwd_dir(nu, st, a);
+      const Site stb = site_coords(sb, a);
+      U.load(nu, sb, A);
+      U.load(mu, sb, B);
+      U.load(nu, fwd_dir(mu, stb, a), W);
+      mm3<false, false>(T, B, W);
+      mm3<true, false>(B, A, T);
+#pragma unroll
+      for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
+      // forward: [U_nu(x) U_mu(x+nu)] U_nu(x+mu)^+
+      U.load(nu, s, A);
+      U.load(mu, fwd_dir(nu, st, a), B);
+      U.load(nu, sf, W);
+      mm3<false, false>(T, A, B);
+      mm3<false, true>(B, T, W);
+#pragma unroll
+      for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
+    }
+    mm3<false, true>(W, S, m);   // W = C U^+
+    C q[9];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) q[3 * r + c] = C{W[3 * r + c].re - W[3 * c + r].re, W[3 * r + c].im + W[3 * c + r].im};
+    const C tr{(q[0].re + q[4].re + q[8].re) / 3.0, (q[0].im + q[4].im + q[8].im) / 3.0};
+#pragma unroll
+    for (int r = 0; r < 3; r++) q[4 * r] = C{q[4 * r].re - tr.re, q[4 * r].im - tr.im};
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[k] = C{0.5 * rho * q[k].im, -0.5 * rho * q[k].re};   // Q = -i rho/2 A
+    stout_exp(q);
+    mm3<false, false>(T, q, m);
+#pragma unroll
+    for (int k = 0; k < 9; k++) m[k] = T[k];
+  }
+  double2 *o = out + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
+#pragma unroll
+  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
+}
+
+int stout_step(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, hipStream_t st) {
+  const DslashArgs a = make_args(in->lat, 0.0, 0.0);
+  const int grid = grid_for(4 * in->lat->V);
+  if (ndim == 3)
+    k_stout<1><<<grid, kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                        (size_t)in->lat->stride, rho);
+  else
+    k_stout<0><<<grid, kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                        (size_t)in->lat->stride, rho);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
 int ape_step(qcd_hip_field out, qcd_hip_field in, double alpha, hipStream_t st) {
   const DslashArgs a = make_args(in->lat, 0.0, 0.0);
   k_ape3d<<<grid_for(4 * in->lat->V), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
@@ -217,6 +368,46 @@ int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nst
   }
   if (!rc && hipStreamSynchronize(st) != hipSuccess) {
     set_error("ape_smear: %s", hipGetErrorString(hipGetLastError()));
+    rc = 2;
+  }
+  qcd_hip_field_destroy(tmp);
+  return rc;
+}
+
+int qcd_hip_stout_smear(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, int nsteps) {
+  QH_TRY(ensure_init());
+  if (!out || !in || out == in || out->kind != QCD_HIP_GAUGE || in->kind != QCD_HIP_GAUGE ||
+      out->prec != QCD_HIP_F64 || in->prec != QCD_HIP_F64 || out->lat->V != in->lat->V || nsteps < 0) {
+    set_error("stout_smear: expected two distinct FP64 gauge fields on one lattice and nsteps >= 0");
+    return 1;
+  }
+  if (ndim != 3 && ndim != 4) {
+    set_error("stout_smear: ndim should be either 3 or 4");
+    return 1;
+  }
+  for (int d = 0; d < 4; d++)
+    if (out->lat->L[d] != in->lat->L[d]) {
+      set_error("stout_smear: lattice extents differ");
+      return 1;
+    }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  hipStream_t st = ctx()->stream;
+  if (nsteps == 0) {
+    QH_CHECK(hipMemcpyAsync(out->data, in->data, out->bytes, hipMemcpyDeviceToDevice, st));
+    QH_CHECK(hipStreamSynchronize(st));
+    return 0;
+  }
+  qcd_hip_field tmp = nullptr;
+  if (nsteps > 1) QH_TRY(qcd_hip_gauge_create(&tmp, in->lat, QCD_HIP_F64));
+  qcd_hip_field src = in;
+  int rc = 0;
+  for (int k = 0; k < nsteps && !rc; k++) {
+    qcd_hip_field dst = ((nsteps - 1 - k) % 2 == 0) ? out : tmp;
+    rc = stout_step(dst, src, rho, ndim, st);
+    src = dst;
+  }
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("stout_smear: %s", hipGetErrorString(hipGetLastError()));
     rc = 2;
   }
   qcd_hip_field_destroy(tmp);

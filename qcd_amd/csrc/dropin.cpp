@@ -385,6 +385,27 @@ int qcd_apeSmear3d(qcd_gaugeField *apeu, qcd_gaugeField *u, qcd_real_8 alpha) {
   return report("qcd_apeSmear3d", rc);
 }
 
+// lib/src/qcd_smearing.c:374-476 (one step; u_out may alias u)
+int qcd_stoutsmearing(qcd_gaugeField *u_out, qcd_gaugeField *u, qcd_real_8 rho, int ndim) {
+  if (!u || !u->initialized || !u_out || !u_out->initialized) {
+    fprintf(stderr, "Error in qcd_stoutsmearing! Gauge fields must be properly initialized\n");
+    return 1;
+  }
+  if (ndim != 3 && ndim != 4) {
+    fprintf(stderr, "ndim should be either 3 or 4");
+    return 1;
+  }
+  qcd_hip_lattice lat;
+  qcd_hip_field g, out = nullptr;
+  int rc = lattice_for(u->geo, &lat);
+  if (!rc) rc = gauge_for(u, lat, &g);
+  if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
+  if (!rc) rc = qcd_hip_stout_smear(out, g, rho, ndim, 1);
+  if (!rc) rc = qcd_hip_gauge_download(out, (double *)u_out->D);
+  qcd_hip_field_destroy(out);
+  return report("qcd_stoutsmearing", rc);
+}
+
 // lib/src/qcd_blas.c:940-1364: spatial links projected onto SU(3) in place
 // (an APE step with alpha = 0)
 void qcd_projectSU33d(qcd_gaugeField *gf) {

@@ -106,3 +106,51 @@ def test_plaquette_device_and_dropin(gpu):
         assert abs(lib.qcd_calculatePlaquette(ctypes.byref(di.gauge(one))) - 1.0) <= 1e-15
     finally:
         di.close()
+
+
+@pytest.mark.parametrize("ndim", [3, 4])
+def test_stout_steps_match_oracle(gpu, ndim):
+    """Stout smearing (lib/src/qcd_smearing.c:374-476) on the GPU against the
+    oracle (pinned by brute force with scipy's expm and gauge covariance)."""
+    Q = gpu
+    u = Q.gauge_generate(L, 0.3, 711)
+    geo = O.Geometry(L)
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat).upload(u)
+    out = Q.Gauge(lat)
+    ref1 = O.stout(geo, u, 0.2, ndim)
+    Q.stout_smear(out, g, 0.2, ndim, 1)
+    assert np.abs(out.download() - ref1).max() <= 1e-13
+    ref2 = O.stout(geo, ref1, 0.2, ndim)
+    Q.stout_smear(out, g, 0.2, ndim, 2)
+    assert np.abs(out.download() - ref2).max() <= 1e-13
+    assert np.array_equal(g.download(), u)
+    # U = 1 stays exactly 1 (the reference's exponential gives NaN there)
+    one = np.zeros_like(u)
+    one[:, :, [0, 1, 2], [0, 1, 2]] = 1.0
+    Q.stout_smear(out, Q.Gauge(lat).upload(one), 0.2, ndim, 1)
+    assert np.array_equal(out.download(), one)
+
+
+def test_dropin_stout_and_bad_arguments(gpu):
+    Q = gpu
+    Ls = (8, 4, 4, 4)
+    u = Q.gauge_generate(Ls, 0.4, 712)
+    di = Q.DropIn(Ls)
+    try:
+        lib = Q.lib()
+        uu, ss = di.gauge(u), di.gauge()
+        assert lib.qcd_stoutsmearing(ctypes.byref(ss), ctypes.byref(uu), 0.1, 4) == 0
+        ref = O.stout(O.Geometry(Ls), u, 0.1, 4)
+        assert np.abs(di.view(ss, 36).reshape(u.shape) - ref).max() <= 1e-13
+        assert lib.qcd_stoutsmearing(ctypes.byref(uu), ctypes.byref(uu), 0.1, 4) == 0   # in place
+        assert np.abs(di.view(uu, 36).reshape(u.shape) - ref).max() <= 1e-13
+        assert lib.qcd_stoutsmearing(ctypes.byref(ss), ctypes.byref(uu), 0.1, 2) == 1
+    finally:
+        di.close()
+    lat = Q.Lattice(Ls)
+    g = Q.Gauge(lat)
+    with pytest.raises(Q.QcdError):
+        Q.stout_smear(g, g, 0.1, 3, 1)
+    with pytest.raises(Q.QcdError):
+        Q.stout_smear(Q.Gauge(lat), g, 0.1, 5, 1)

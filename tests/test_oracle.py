@@ -388,3 +388,101 @@ def test_ape_staples_brute_force_and_gauge_covariance():
     up = np.ascontiguousarray(gauge_transform(L, u, g))
     ap = O.ape3d(geo, up, alpha)
     assert np.abs(ap - gauge_transform(L, a, g)).max() <= 1e-12
+
+
+def herm_traceless(rng, scale):
+    h = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    h = (h + h.conj().T) / 2
+    return scale * (h - np.trace(h) / 3 * np.eye(3))
+
+
+def test_stout_exponential_matches_expm():
+    """exp(iQ) by Peardon-Morningstar (qcd_smearing.c:293-370) against scipy's
+    Pade expm, over six decades of |Q|, both signs of tr Q^3 (the conjugated
+    branch, eq. 34), degenerate spectra (theta = 0), the small-w series of
+    xi0 and Q = 0 (where the reference divides 0 by 0)."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(41)
+    qs = []
+    for scale in (1e-9, 1e-4, 0.03, 0.3, 1.0, 3.0):
+        for _ in range(6):
+            q = herm_traceless(rng, scale)
+            qs += [q, -q]
+    for a in (1e-3, 0.2, 1.5):
+        qs += [np.diag([a, a, -2 * a]).astype(complex), np.diag([-a, -a, 2 * a]).astype(complex)]
+    qs.append(np.zeros((3, 3), dtype=complex))
+    got = O.stout_exp(np.array(qs))
+    for q, e in zip(qs, got):
+        assert np.abs(e - expm(1j * q)).max() <= 1e-13
+        assert np.abs(e.conj().T @ e - np.eye(3)).max() <= 1e-14
+
+
+def stout_brute(L, u, rho, ndim):
+    """The stout step written with numpy and scipy's expm: staples, the
+    traceless anti-Hermitian part of C U^+, U' = exp(iQ) U."""
+    from scipy.linalg import expm
+    T, X, Y, Z = L
+    c = coords(L)
+    start = 1 if ndim == 3 else 0
+
+    def nb(l, mu, d):
+        cc = c[l].copy()
+        cc[mu] = (cc[mu] + d) % L[mu]
+        return int(cc[0] + T * (cc[1] + X * (cc[2] + Y * cc[3])))
+
+    out = u.copy()
+    for l in range(u.shape[0]):
+        for mu in range(start, 4):
+            s = np.zeros((3, 3), dtype=complex)
+            for nu in range(start, 4):
+                if nu == mu:
+                    continue
+                b = nb(l, nu, -1)
+                s += u[b, nu].conj().T @ u[b, mu] @ u[nb(b, mu, 1), nu]
+                s += u[l, nu] @ u[nb(l, nu, 1), mu] @ u[nb(l, mu, 1), nu].conj().T
+            w = s @ u[l, mu].conj().T
+            a = w - w.conj().T
+            a -= np.trace(a) / 3 * np.eye(3)
+            q = -0.5j * rho * a
+            out[l, mu] = expm(1j * q) @ u[l, mu]
+    return out
+
+
+@pytest.mark.parametrize("ndim", [3, 4])
+def test_stout_brute_force_covariance_and_fixed_point(ndim):
+    L = (2, 4, 2, 4)
+    geo = O.Geometry(L)
+    V = geo.V
+    u = O.gauge_generate(L, 0.8, 51)
+    rho = 0.15
+    s = O.stout(geo, u, rho, ndim)
+    assert np.abs(s - stout_brute(L, u, rho, ndim)).max() <= 1e-13
+    if ndim == 3:
+        assert np.array_equal(s[:, 0], u[:, 0])   # temporal links untouched
+    for l in range(V):
+        for mu in range(4):
+            assert np.abs(s[l, mu].conj().T @ s[l, mu] - np.eye(3)).max() <= 1e-14
+            assert abs(np.linalg.det(s[l, mu]) - 1) <= 1e-13
+    # covariance: smearing the transformed field = transforming the smeared field
+    g = random_su3(V, 52)
+    sp = O.stout(geo, np.ascontiguousarray(gauge_transform(L, u, g)), rho, ndim)
+    assert np.abs(sp - gauge_transform(L, s, g)).max() <= 1e-12
+    # U = 1 is a fixed point (exactly: Q = 0 takes the series branch), and rho = 0 changes nothing
+    one = unit_gauge(V)
+    assert np.array_equal(O.stout(geo, one, rho, ndim), one)
+    assert np.abs(O.stout(geo, u, 0.0, ndim) - u).max() <= 1e-15
+    with pytest.raises(ValueError):
+        O.stout(geo, u, rho, 2)
+
+
+def test_stout_smooths():
+    """Small rho moves the links along the gradient of the plaquette action:
+    the 4d plaquette rises with each step."""
+    L = (4, 4, 4, 4)
+    geo = O.Geometry(L)
+    u = O.gauge_generate(L, 0.6, 53)
+    p = [O.plaquette(geo, u)]
+    for _ in range(3):
+        u = O.stout(geo, u, 0.1, 4)
+        p.append(O.plaquette(geo, u))
+    assert p[0] < p[1] < p[2] < p[3] <= 1.0
