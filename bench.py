@@ -66,7 +66,8 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
     return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
-            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march, two lanes per site"][cfg[0] & 0xff],
+            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march over LDS-staged row tiles",
+                                      "z-march, forward z neighbour carried in LDS"][cfg[0] & 0xff],
                        "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
                        "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}}
 

@@ -88,12 +88,14 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
 
 /* Autotuner: time every traversal (0 = one contiguous range per XCD,
  * 1 = one y-slab per XCD swept along z, 2 | (n << 8) = each lane marches n
- * steps along z carrying the centre spinor and the backward z hop in
- * registers) and occupancy (0 = compiler default,
- * 3 or 4 minimum waves per SIMD) on these fields and keep the fastest for
- * their (lattice extents, precisions).  Runs by itself on first use unless
- * the environment sets QCD_HIP_AUTOTUNE=0.  All candidates give bit-identical
- * results. */
+ * steps along z carrying the backward z hop in registers, 3 | (n << 8) = the
+ * same over tiles of whole t-rows staged in LDS with their x halo rows,
+ * 4 | (n << 8) = traversal 2 with the forward z neighbour kept in LDS as the
+ * next step's centre) and occupancy variant (0 = compiler default, 2 or 3 =
+ * at least that many waves per SIMD, 16 = scheduling fences between hop
+ * directions) on these fields and keep the fastest for their (lattice
+ * extents, precisions).  Runs by itself on first use unless the environment
+ * sets QCD_HIP_AUTOTUNE=0.  All candidates give bit-identical results. */
 int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, int tuned[2], double *ms);
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ);
 

@@ -103,11 +103,16 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
 // before, so it comes from cache).  Each step starts with the local terms
 // (mass, both z hops) so the centre spinor dies before the t/x/y hops load.
 // acc holds -2 diag(psi) + sum of hops, so out = -1/2 acc.
-template <int PS, int PG, int EPI, bool RED, int OCC>
+// LDSC (MAP_ZMARCH_LDS): the forward z neighbour's stored values go to this
+// lane's LDS slot and come back as the next step's centre (same bits).
+template <int PS, int PG, int EPI, bool RED, int OCC, bool LDSC = false>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
             double *__restrict__ partials) {
   using R = typename Prec<PS>::R;
+  using T = typename Prec<PS>::T;
+  __shared__ T zfwd[LDSC ? 12 : 1][kBlock];
+  __shared__ float zfwd_n[LDSC && PS == QCD_HIP_F16 ? kBlock : 1];
   const BlockRow br = block_row(a);
   const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, br.row);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
@@ -141,10 +146,16 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
       const unsigned s = st.s;
       Cx<R> acc[4][3];
       {
-        typename Prec<PS>::T rc[12];
+        T rc[12];
         typename Prec<PG>::T u[9];
         float nc = 0.f;
-        in.load_raw(s, rc, nc);
+        if (LDSC && z > z0) {
+#pragma unroll
+          for (int k = 0; k < 12; k++) rc[k] = zfwd[k][threadIdx.x];
+          if constexpr (PS == QCD_HIP_F16) nc = zfwd_n[threadIdx.x];
+        } else {
+          in.load_raw(s, rc, nc);
+        }
         local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
         U.load_raw(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
@@ -153,6 +164,11 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
         Cx<R> g0[3], g1[3];
         float nf = 0.f;
         in.load_raw(sf, rc, nf);
+        if constexpr (LDSC) {
+#pragma unroll
+          for (int k = 0; k < 12; k++) zfwd[k][threadIdx.x] = rc[k];
+          if constexpr (PS == QCD_HIP_F16) zfwd_n[threadIdx.x] = nf;
+        }
         hop_half_raw<3, -1, false, PS, PG>(g0, g1, rc, nf, u, php);
         rebuild_add<3, -1>(acc, g0, g1);
       }
@@ -328,11 +344,14 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
   // z-march traversals interleave the batch rows into grid.x (see block_row)
-  const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE;
+  const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE || a.map == MAP_ZMARCH_LDS;
   const dim3 gr(zmarch ? (unsigned)grid * a.count : (unsigned)grid, zmarch ? 1u : (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
     k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
                                                                   SpinorOut<PS>::of(out, ro), bi, partials);
+  else if (a.map == MAP_ZMARCH_LDS)
+    k_dslash_zm<PS, PG, EPI, RED, OCC, true><<<gr, kBlock, 0, stream>>>(
+        a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g), SpinorOut<PS>::of(out, ro), bi, partials);
   else if (a.map == MAP_ZTILE) {
     const size_t lds = ztile_lds_bytes(PS, a.L[0], a.xb);
     static std::atomic<size_t> lds_set{0};   // per instantiation: raise the dynamic LDS limit once
@@ -429,10 +448,10 @@ int set_traversal(DslashArgs &a, int map, int ps) {
     }
     map = MAP_ZMARCH | (map & ~0xff);
   }
-  if ((map & 0xff) == MAP_ZMARCH) {
+  if ((map & 0xff) == MAP_ZMARCH || (map & 0xff) == MAP_ZMARCH_LDS) {
     const unsigned zlen = std::max(1u, std::min((unsigned)(map >> 8), a.L[3]));
     const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
-    a.map = MAP_ZMARCH;
+    a.map = map & 0xff;
     a.zlen = zlen;
     a.col_blocks = (P3 + kBlock - 1) / kBlock;
     const unsigned nz = (a.L[3] + zlen - 1) / zlen;
@@ -531,7 +550,10 @@ static const int kMaps[] = {MAP_ZMARCH | (1 << 8), MAP_ZMARCH | (2 << 8), MAP_ZM
                             MAP_ZMARCH | (12 << 8), MAP_ZMARCH | (16 << 8),
                             MAP_ZTILE | (1 << 8),  MAP_ZTILE | (2 << 8),  MAP_ZTILE | (3 << 8),
                             MAP_ZTILE | (4 << 8),  MAP_ZTILE | (6 << 8),  MAP_ZTILE | (8 << 8),
-                            MAP_ZTILE | (12 << 8), MAP_ZTILE | (16 << 8)};
+                            MAP_ZTILE | (12 << 8), MAP_ZTILE | (16 << 8),
+                            MAP_ZMARCH_LDS | (2 << 8), MAP_ZMARCH_LDS | (3 << 8), MAP_ZMARCH_LDS | (4 << 8),
+                            MAP_ZMARCH_LDS | (6 << 8), MAP_ZMARCH_LDS | (8 << 8), MAP_ZMARCH_LDS | (12 << 8),
+                            MAP_ZMARCH_LDS | (16 << 8)};
 static const int kOccs[] = {0, 2, 3, 16};
 
 int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force) {
@@ -731,7 +753,7 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ) {
   QH_TRY(ensure_init());
   const int m0 = map & 0xff;
-  if (!in || !gauge || m0 < 0 || m0 > 3 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
+  if (!in || !gauge || m0 < 0 || m0 > 4 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
     set_error("dslash_set_config: bad arguments");
     return 1;
   }

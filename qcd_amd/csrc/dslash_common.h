@@ -224,7 +224,11 @@ struct GaugeIn {
 //              consecutive x-rows (all t) of one y and stages them, with one
 //              halo row on each side, in LDS every step, so t and x
 //              neighbours never go through the vector memory path.
-enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3 };
+//  MAP_ZMARCH_LDS: MAP_ZMARCH whose lanes keep the forward z neighbour,
+//              loaded for one step's z hop, in an LDS slot of their own and
+//              read it back as the next step's centre spinor instead of
+//              loading it again through L2.
+enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3, MAP_ZMARCH_LDS = 4 };
 
 struct DslashArgs {
   unsigned V;

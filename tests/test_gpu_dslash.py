@@ -169,7 +169,8 @@ def test_launch_configs_bit_identical(gpu):
         x = Q.Spinor(lat, p).upload(psi)
         y = Q.Spinor(lat, p)
         by_kernel = {}
-        for map_ in (0, 1, 2 | (1 << 8), 2 | (3 << 8), 2 | (5 << 8), 3 | (1 << 8), 3 | (2 << 8), 3 | (5 << 8)):
+        for map_ in (0, 1, 2 | (1 << 8), 2 | (3 << 8), 2 | (5 << 8), 3 | (1 << 8), 3 | (2 << 8), 3 | (5 << 8),
+                     4 | (1 << 8), 4 | (3 << 8), 4 | (8 << 8)):
             for occ in (0, 2, 3, 16):
                 Q.dslash_set_config(x, g, map_, occ)
                 Q.dslash(y, x, g, m, 0.01, Q.OP_Q)
