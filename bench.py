@@ -72,6 +72,33 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
                        "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}}
 
 
+def multi_rhs_dslash_leg(Q, L, prec, nrhs, steps, warmup, seed=3101):
+    """One Dslash over nrhs right-hand sides (the batched launch: each site
+    block's links are read once through one XCD's L2 for all of them).
+    Algorithmic bytes per site: nrhs * (spinor in + out) + links."""
+    V = int(np.prod(L))
+    lat = Q.Lattice(L)
+    g64 = Q.Gauge(lat)
+    Q.gauge_generate_device(g64, 0.3, seed)
+    p = Q.PREC[prec]
+    g = g64
+    if p != Q.F64:
+        g = Q.Gauge(lat, p)
+        Q.convert(g, g64)
+    psi = Q.spinor_gaussian(L, seed + 1)
+    x = Q.Spinor(lat, p, nrhs=nrhs)
+    for r in range(nrhs):
+        x.upload(psi, rhs=r)
+    y = Q.Spinor(lat, p, nrhs=nrhs)
+    m = 1 / (2 * 0.132) - 4
+    Q.dslash_tune(y, x, g)
+    ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
+    spinor, links = {"f64": (192, 576), "f32": (96, 288), "f16": (52, 144)}[prec]
+    bps = nrhs * 2 * spinor + links
+    return {"rhs": nrhs, "ms": ms, "us_per_rhs": ms * 1e3 / nrhs, "bytes_per_site": bps,
+            "GBps": V * bps / (ms * 1e-3) / 1e9}
+
+
 def pmc_traffic(L, prec, cfg, timeout=150):
     """HBM bytes per Dslash launch, from two rocprofv3 --pmc passes (FETCH_SIZE,
     then WRITE_SIZE) of qcd_amd.profile_run on the same lattice, precision and
@@ -181,27 +208,31 @@ def config5_leg(Q, batch=4, seed=5005):
     t_smear = time.perf_counter() - t0
     del ga
     x = Q.Spinor(lat, nrhs=12)
-    S = Q.Solver(lat, g, n_restart=32, tol=1e-10, inner_prec="f16", batch=batch, max_outer=20000, eo=1)
     b1 = Q.Spinor(lat).upload(Q.point_source(L, (0, 0, 0, 0), 0, 0))
     x1 = Q.Spinor(lat)
     Q.Solver(lat, g, n_restart=32, tol=1e-2, inner_prec="f16", batch=1, eo=1).solve(x1, b1, kappa)
     del b1, x1
-    Q.synchronize()
-    t0 = time.perf_counter()
-    st = S.solve(x, b, kappa)
-    Q.synchronize()
-    t_solve = time.perf_counter() - t0
-    worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
-    assert worst <= 1e-10, f"config 5 solve not converged: {worst}"
-    return {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
-                      "Gauss-smeared (alpha 4.0, 50 iterations, FP64), mixed FP16-inner even-odd solve, tol 1e-10",
-            "ape_seconds": t_ape, "smear_seconds": t_smear, "solve_seconds": t_solve,
-            "solves_per_s": 12 / t_solve, "batch": batch, "gcr_iterations": st["inner_iters"],
-            "outer_cycles": st["outer"], "worst_b_minus_Dx": worst,
-            "pipeline_seconds": t_ape + t_smear + t_solve}
+    solves = {}
+    for n in (32, 12):   # invert.ini:8's N_restart, and the shorter cycle
+        S = Q.Solver(lat, g, n_restart=n, tol=1e-10, inner_prec="f16", batch=batch, max_outer=20000, eo=1)
+        Q.synchronize()
+        t0 = time.perf_counter()
+        st = S.solve(x, b, kappa)
+        Q.synchronize()
+        t_solve = time.perf_counter() - t0
+        S.close()
+        worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
+        assert worst <= 1e-10, f"config 5 solve not converged: {worst}"
+        solves[n] = {"solve_seconds": t_solve, "solves_per_s": 12 / t_solve, "gcr_iterations": st["inner_iters"],
+                     "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
+    out = {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
+                     "Gauss-smeared (alpha 4.0, 50 iterations, FP64), mixed FP16-inner even-odd solve, tol 1e-10",
+           "ape_seconds": t_ape, "smear_seconds": t_smear, "batch": batch, **solves[32],
+           "pipeline_seconds": t_ape + t_smear + solves[32]["solve_seconds"], "gcr12": solves[12]}
+    return out
 
 
-def serve_stream(n, rate, seed):
+def serve_stream(n, rate, seed, n_restart=32):
     """Poisson arrivals; volumes, kappa, eps and rhs counts drawn per request."""
     rng = np.random.default_rng(seed)
     vols = [(16, 8, 8, 8), (32, 16, 16, 16), (48, 24, 24, 24)]
@@ -210,7 +241,7 @@ def serve_stream(n, rate, seed):
         t += rng.exponential(1.0 / rate)
         reqs.append({"arrival_s": t, "L": vols[rng.integers(3)], "kappa": float(rng.uniform(0.120, 0.132)),
                      "eps": float(rng.uniform(0.2, 0.35)), "seed": int(10000 + k),
-                     "nrhs": int(rng.choice([1, 12])), "n_restart": 32, "tol": 1e-10})
+                     "nrhs": int(rng.choice([1, 12])), "n_restart": n_restart, "tol": 1e-10})
     return reqs
 
 
@@ -301,6 +332,11 @@ def main():
     big = dslash_leg(Q, (96, 48, 48, 48), "f64", max(10, args.steps // 10), max(2, args.warmup // 10))
     dsl["f64_48c96"] = big
     log(f"f64 dslash 48^3x96: {big['ms']*1e3:.1f} us, {big['GBps']:.0f} GB/s")
+    for nr in (16, 48):
+        mr = multi_rhs_dslash_leg(Q, L, "f16", nr, max(10, args.steps // 10), max(2, args.warmup // 10))
+        dsl[f"f16_{nr}rhs"] = mr
+        log(f"f16 dslash 24^3x48, {nr} rhs: {mr['us_per_rhs']:.1f} us per rhs, {mr['GBps']:.0f} GB/s "
+            f"({mr['bytes_per_site']} B/site)")
 
     traffic, traffic_note = None, "not measured (--no-pmc)"
     if not args.no_pmc:
@@ -348,7 +384,8 @@ def main():
             solves["config5"] = config5_leg(Q, batch=6)
             c5 = solves["config5"]
             log(f"config5 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
-                f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s)")
+                f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s); GCR(12) solve "
+                f"{c5['gcr12']['solve_seconds']:.2f}s ({c5['gcr12']['solves_per_s']:.2f} solves/s)")
 
     serve = None
     if not args.no_serve:
@@ -357,10 +394,12 @@ def main():
         serve = {"stream": f"{args.serve_requests} requests, Poisson rate {args.serve_rate}/s, volumes "
                            "{8^3x16,16^3x32,24^3x48}, kappa U[0.120,0.132], eps U[0.2,0.35], rhs {1,12}, "
                            "4 workers, 200 GB budget, tol 1e-10, even-odd preconditioned inner solves"}
-        for prec in ("f16", "f64"):
-            serve["mixed_f16" if prec == "f16" else "fp64"] = serve_leg(Q, reqs, prec)
-            s = serve["mixed_f16" if prec == "f16" else "fp64"]
-            log(f"serve {prec}: {s['solves_per_s']:.2f} solves/s p50 {s['p50_latency_s']:.2f}s "
+        reqs12 = serve_stream(args.serve_requests, args.serve_rate, 777, n_restart=12)
+        for name, prec, rq in (("mixed_f16", "f16", reqs), ("fp64", "f64", reqs), ("mixed_f16_gcr12", "f16", reqs12),
+                               ("fp64_gcr12", "f64", reqs12)):
+            serve[name] = serve_leg(Q, rq, prec)
+            s = serve[name]
+            log(f"serve {name}: {s['solves_per_s']:.2f} solves/s p50 {s['p50_latency_s']:.2f}s "
                 f"p99 {s['p99_latency_s']:.2f}s wall {s['wall_s']:.1f}s")
 
     cpu = None
