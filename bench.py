@@ -177,13 +177,13 @@ def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0
             "worst_b_minus_Dx": worst}
 
 
-def config3_sites(L=(48, 24, 24, 24), seed=3003):
+def source_sites_24c48(L=(48, 24, 24, 24), seed=3003):
     """Four source sites drawn from a seeded generator."""
     rng = np.random.default_rng(seed)
     return [tuple(int(rng.integers(0, L[d])) for d in range(4)) for _ in range(4)]
 
 
-def config5_leg(Q, batch=4, seed=5005):
+def pipeline_48c96_leg(Q, batch=4, seed=5005):
     """48^3x96, eps 0.3, kappa 0.132: 10 APE steps (alpha 0.5) of the links, 12
     point sources at the origin Gauss-smeared (alpha 4.0, 50 iterations, FP64)
     with the APE links, then a 12-rhs mixed solve (FP16 inner, even-odd) with
@@ -222,7 +222,7 @@ def config5_leg(Q, batch=4, seed=5005):
         t_solve = time.perf_counter() - t0
         S.close()
         worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
-        assert worst <= 1e-10, f"config 5 solve not converged: {worst}"
+        assert worst <= 1e-10, f"48^3x96 solve not converged: {worst}"
         solves[n] = {"solve_seconds": t_solve, "solves_per_s": 12 / t_solve, "gcr_iterations": st["inner_iters"],
                      "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
     out = {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
@@ -305,7 +305,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--no-config5", action="store_true", help="skip the 48^3x96 APE + smearing + solve pipeline")
+    ap.add_argument("--no-pipeline48", action="store_true", help="skip the 48^3x96 APE + smearing + solve pipeline")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the headline kernel's HBM traffic")
     args = ap.parse_args()
@@ -365,25 +365,25 @@ def main():
             log(f"solves {name}: {solves[name]['solves_per_s']:.2f}/s ({solves[name]['gcr_iterations']} GCR iterations)")
         solves["speedup_mixed_f16_eo_batched_vs_fp64_eo_n12"] = (solves["mixed_f16_eo_batched12_n12"]["solves_per_s"] /
                                                                  solves["fp64_eo_one_at_a_time_n12"]["solves_per_s"])
-        c3 = config3_sites()
+        c3 = source_sites_24c48()
         L3 = (48, 24, 24, 24)
-        solves["config3"] = {"config": "24^3x48, kappa 0.132, eps 0.3, 48 rhs = 12 point sources at each of 4 "
+        solves["multi_source_24c48"] = {"config": "24^3x48, kappa 0.132, eps 0.3, 48 rhs = 12 point sources at each of 4 "
                                        f"seeded sites {c3}, tol 1e-10 (FP64)"}
         for name, prec, batch, n in (("fp64_eo_one_at_a_time", "f64", 1, 32), ("mixed_f16_eo_batched48", "f16", 48, 32),
                                      ("fp64_eo_one_at_a_time_n12", "f64", 1, 12),
                                      ("mixed_f16_eo_batched48_n12", "f16", 48, 12)):
-            solves["config3"][name] = solves_leg(Q, prec, batch, n, L=L3, kappa=0.132, eps=0.3, seed=3003, sites=c3,
+            solves["multi_source_24c48"][name] = solves_leg(Q, prec, batch, n, L=L3, kappa=0.132, eps=0.3, seed=3003, sites=c3,
                                                  eo=1)
-            log(f"solves config3 {name}: {solves['config3'][name]['solves_per_s']:.2f}/s")
-        solves["config3"]["speedup_mixed_vs_fp64"] = (solves["config3"]["mixed_f16_eo_batched48"]["solves_per_s"] /
-                                                      solves["config3"]["fp64_eo_one_at_a_time"]["solves_per_s"])
-        solves["config3"]["speedup_mixed_vs_fp64_n12"] = (
-            solves["config3"]["mixed_f16_eo_batched48_n12"]["solves_per_s"] /
-            solves["config3"]["fp64_eo_one_at_a_time_n12"]["solves_per_s"])
-        if not args.no_config5:
-            solves["config5"] = config5_leg(Q, batch=6)
-            c5 = solves["config5"]
-            log(f"config5 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
+            log(f"solves 24^3x48 48 rhs {name}: {solves['multi_source_24c48'][name]['solves_per_s']:.2f}/s")
+        solves["multi_source_24c48"]["speedup_mixed_vs_fp64"] = (solves["multi_source_24c48"]["mixed_f16_eo_batched48"]["solves_per_s"] /
+                                                      solves["multi_source_24c48"]["fp64_eo_one_at_a_time"]["solves_per_s"])
+        solves["multi_source_24c48"]["speedup_mixed_vs_fp64_n12"] = (
+            solves["multi_source_24c48"]["mixed_f16_eo_batched48_n12"]["solves_per_s"] /
+            solves["multi_source_24c48"]["fp64_eo_one_at_a_time_n12"]["solves_per_s"])
+        if not args.no_pipeline48:
+            solves["pipeline_48c96"] = pipeline_48c96_leg(Q, batch=6)
+            c5 = solves["pipeline_48c96"]
+            log(f"48^3x96 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
                 f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s); GCR(12) solve "
                 f"{c5['gcr12']['solve_seconds']:.2f}s ({c5['gcr12']['solves_per_s']:.2f} solves/s)")
 

@@ -13,7 +13,7 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CONFIG1 = (16, 8, 8, 8)   # 8^3 x 16, tests/test_wilson.c pattern
+L_8C16 = (16, 8, 8, 8)   # 8^3 x 16, tests/test_wilson.c pattern
 
 
 def _fields(L, eps, su, sp):
@@ -23,9 +23,9 @@ def _fields(L, eps, su, sp):
 
 @pytest.mark.parametrize("op", ["D", "Q"])
 @pytest.mark.parametrize("tm", [0.0, 0.05])
-def test_fp64_config1_parity(gpu, op, tm):
+def test_fp64_8c16_parity(gpu, op, tm):
     Q = gpu
-    L = CONFIG1
+    L = L_8C16
     kappa = 0.125
     m = 1 / (2 * kappa) - 4
     u, psi = _fields(L, 0.2, 11, 12)
@@ -59,7 +59,7 @@ def test_dropin_apply_matches_oracle(gpu):
     """qcd_applyWilsonTMOp / qcd_applyQTMOp through host qcd_vector structs."""
     import ctypes
     Q = gpu
-    L = CONFIG1
+    L = L_8C16
     kappa, tm = 0.125, 0.01
     m = 1 / (2 * kappa) - 4
     u, psi = _fields(L, 0.2, 31, 32)
@@ -93,7 +93,7 @@ def test_c_caller_program(gpu, tmp_path):
     import os
     import subprocess
     from conftest import ROOT
-    L = CONFIG1
+    L = L_8C16
     out = tmp_path / "dpsi.bin"
     exe = os.path.join(ROOT, "build", "bin", "wilson_dropin")
     r = subprocess.run([exe, *map(str, L), "0.125", "0.0", "0.2", "41", "42", str(out)],
@@ -111,7 +111,7 @@ def test_c_caller_program(gpu, tmp_path):
 def test_low_precision_operator(gpu, prec, tol):
     """FP32 / FP16-storage (per-site norm) Dslash vs the FP64 oracle."""
     Q = gpu
-    L = CONFIG1
+    L = L_8C16
     m = 1 / (2 * 0.13) - 4
     u, psi = _fields(L, 0.3, 51, 52)
     lat = Q.Lattice(L)
