@@ -194,6 +194,133 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
 
+// both hops of direction MU for two right-hand sides: each link is loaded
+// once; per right-hand side the arithmetic is that of hop_dir
+template <int MU, int PS, int PG, typename R>
+__device__ __forceinline__ void hop_dir2(Cx<R> acc0[4][3], Cx<R> acc1[4][3], const DslashArgs &a, const Site &st,
+                                         const SpinorIn<PS> &in0, const SpinorIn<PS> &in1, const GaugeIn<PG> &U) {
+  const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
+  const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
+  typename Prec<PS>::T r[12];
+  typename Prec<PG>::T u[9];
+  Cx<R> g0[3], g1[3];
+  float n = 0.f;
+  {
+    const unsigned sf = neighbour<MU, +1>(st, a);
+    U.load_raw(MU, st.s, u);
+    in0.load_raw(sf, r, n);
+    hop_half_raw<MU, -1, false, PS, PG>(g0, g1, r, n, u, php);
+    rebuild_add<MU, -1>(acc0, g0, g1);
+    in1.load_raw(sf, r, n);
+    hop_half_raw<MU, -1, false, PS, PG>(g0, g1, r, n, u, php);
+    rebuild_add<MU, -1>(acc1, g0, g1);
+  }
+  {
+    const unsigned sb = neighbour<MU, -1>(st, a);
+    U.load_raw(MU, sb, u);
+    in0.load_raw(sb, r, n);
+    hop_half_raw<MU, +1, true, PS, PG>(g0, g1, r, n, u, phm);
+    rebuild_add<MU, +1>(acc0, g0, g1);
+    in1.load_raw(sb, r, n);
+    hop_half_raw<MU, +1, true, PS, PG>(g0, g1, r, n, u, phm);
+    rebuild_add<MU, +1>(acc1, g0, g1);
+  }
+}
+
+// out = -1/2 acc with the D or Q epilogue, stored at site s
+template <int EPI, int PS, typename R>
+__device__ __forceinline__ void finish_site(const Cx<R> acc[4][3], const SpinorOut<PS> &out, const SpinorIn<PS> &b,
+                                            unsigned s) {
+  Cx<R> o[12];
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[3 * sp + c] = Cx<R>{R(-0.5) * acc[sp][c].re, R(-0.5) * acc[sp][c].im};
+  apply_epilogue<EPI>(o, b, s);
+  out.store(s, o);
+}
+
+// MAP_ZMARCH2: the z-march of k_dslash_zm with two right-hand sides per lane
+// (batch rows 2p and 2p+1 of pair p; an odd batch's last pair repeats its
+// first row and stores it once).  Same per-rhs arithmetic and order as
+// k_dslash_zm, so the results are bit-identical.  D and Q epilogues only.
+template <int PS, int PG, int EPI, int OCC>
+__global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
+k_dslash_zm2(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all) {
+  using R = typename Prec<PS>::R;
+  using T = typename Prec<PS>::T;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned p = lb % a.pairs, blk = lb / a.pairs;
+  const unsigned r0 = 2 * p, r1 = min(2 * p + 1, a.count - 1);
+  const bool two = r1 != r0;
+  const RhsView<PS> v0 = rhs_view(a, in_all, out_all, b_all, r0);
+  const RhsView<PS> v1 = rhs_view(a, in_all, out_all, b_all, r1);
+  const unsigned cb = blk % a.col_blocks, zc = blk / a.col_blocks;
+  const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
+  const unsigned col = cb * kBlock + threadIdx.x;
+  const unsigned z0 = zc * a.zlen;
+  const unsigned z1 = min(z0 + a.zlen, a.L[3]);
+  if (col >= P3 || z0 >= a.L[3]) return;
+  Site st;
+  st.valid = true;
+  st.x = col / a.L[0];
+  st.t = col - st.x * a.L[0];
+  st.y = st.x / a.L[1];
+  st.x -= st.y * a.L[1];
+  const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
+  const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
+  const R mp4 = R(a.mp4), tm = R(a.tm);
+  Cx<R> cz0[2][3], cz1[2][3];
+  {
+    const unsigned zm = z0 == 0 ? a.L[3] - 1 : z0 - 1;
+    const unsigned sm = col + P3 * zm;
+    typename Prec<PG>::T u[9];
+    T r[12];
+    float n = 0.f;
+    U.load_raw(3, sm, u);
+    v0.in.load_raw(sm, r, n);
+    hop_half_raw<3, +1, true, PS, PG>(cz0[0], cz1[0], r, n, u, phm);
+    v1.in.load_raw(sm, r, n);
+    hop_half_raw<3, +1, true, PS, PG>(cz0[1], cz1[1], r, n, u, phm);
+  }
+  for (unsigned z = z0; z < z1; z++) {
+    st.z = z;
+    st.s = col + P3 * z;
+    const unsigned s = st.s;
+    Cx<R> acc0[4][3], acc1[4][3];
+    {
+      T rc[12];
+      typename Prec<PG>::T u[9];
+      float nc = 0.f;
+      Cx<R> g0[3], g1[3];
+      U.load_raw(3, s, u);
+      const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
+      v0.in.load_raw(s, rc, nc);
+      local_terms<PS>(acc0, rc, nc, cz0[0], cz1[0], mp4, tm);
+      hop_half_raw<3, +1, true, PS, PG>(cz0[0], cz1[0], rc, nc, u, phm);
+      v0.in.load_raw(sf, rc, nc);
+      hop_half_raw<3, -1, false, PS, PG>(g0, g1, rc, nc, u, php);
+      rebuild_add<3, -1>(acc0, g0, g1);
+      v1.in.load_raw(s, rc, nc);
+      local_terms<PS>(acc1, rc, nc, cz0[1], cz1[1], mp4, tm);
+      hop_half_raw<3, +1, true, PS, PG>(cz0[1], cz1[1], rc, nc, u, phm);
+      v1.in.load_raw(sf, rc, nc);
+      hop_half_raw<3, -1, false, PS, PG>(g0, g1, rc, nc, u, php);
+      rebuild_add<3, -1>(acc1, g0, g1);
+    }
+    constexpr bool F = (OCC & 16) != 0;
+    if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+    hop_dir2<0, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+    if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+    hop_dir2<1, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+    if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+    hop_dir2<2, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+    if constexpr (F) __builtin_amdgcn_sched_barrier(0);
+    finish_site<EPI>(acc0, v0.out, v0.b, s);
+    if (two) finish_site<EPI>(acc1, v1.out, v1.b, s);
+  }
+}
+
 // Bytes of dynamic LDS a MAP_ZTILE workgroup uses: (xb + 2) staged rows of T
 // sites, each site its 12 stored components and (FP16) its norm.
 inline size_t ztile_lds_bytes(int ps, unsigned T, unsigned xb) {
@@ -344,7 +471,7 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
   // z-march traversals interleave the batch rows into grid.x (see block_row)
-  const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE || a.map == MAP_ZMARCH_LDS;
+  const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE || a.map == MAP_ZMARCH_LDS || a.map == MAP_ZMARCH2;
   const dim3 gr(zmarch ? (unsigned)grid * a.count : (unsigned)grid, zmarch ? 1u : (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
     k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
@@ -352,6 +479,15 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
   else if (a.map == MAP_ZMARCH_LDS)
     k_dslash_zm<PS, PG, EPI, RED, OCC, true><<<gr, kBlock, 0, stream>>>(
         a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g), SpinorOut<PS>::of(out, ro), bi, partials);
+  else if (a.map == MAP_ZMARCH2) {
+    if constexpr (PS != QCD_HIP_F64 && !RED && (EPI == EPI_D || EPI == EPI_Q))
+      k_dslash_zm2<PS, PG, EPI, OCC><<<dim3((unsigned)grid * a.pairs), kBlock, 0, stream>>>(
+          a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g), SpinorOut<PS>::of(out, ro), bi);
+    else {
+      set_error("dslash: two-rhs traversal needs FP32/FP16, the D or Q epilogue and no reduction");
+      return 1;
+    }
+  }
   else if (a.map == MAP_ZTILE) {
     const size_t lds = ztile_lds_bytes(PS, a.L[0], a.xb);
     static std::atomic<size_t> lds_set{0};   // per instantiation: raise the dynamic LDS limit once
@@ -448,7 +584,7 @@ int set_traversal(DslashArgs &a, int map, int ps) {
     }
     map = MAP_ZMARCH | (map & ~0xff);
   }
-  if ((map & 0xff) == MAP_ZMARCH || (map & 0xff) == MAP_ZMARCH_LDS) {
+  if ((map & 0xff) == MAP_ZMARCH || (map & 0xff) == MAP_ZMARCH_LDS || (map & 0xff) == MAP_ZMARCH2) {
     const unsigned zlen = std::max(1u, std::min((unsigned)(map >> 8), a.L[3]));
     const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
     a.map = map & 0xff;
@@ -484,6 +620,7 @@ struct TuneKey {
 struct TuneVal {
   int map, occ;
   float ms;
+  int bmap = 0, bocc = 0;   // launches over two or more rhs (0: use map, occ)
 };
 static std::vector<std::pair<TuneKey, TuneVal>> g_tune;
 static int g_autotune = -1;   // -1: read QCD_HIP_AUTOTUNE once
@@ -525,6 +662,11 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
   a.out_rs = (unsigned)bt.out_rs;
   a.b_rs = (unsigned)bt.b_rs;
   a.nblk = (unsigned)grid;
+  a.pairs = (a.count + 1) / 2;
+  if (a.map == MAP_ZMARCH2 && (a.count < 2 || in->prec == QCD_HIP_F64 || red || (epi != EPI_D && epi != EPI_Q))) {
+    // the two-rhs traversal does not apply: same z-march, one rhs per lane
+    a.map = MAP_ZMARCH;
+  }
   if (grid_out) *grid_out = grid;
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
@@ -584,6 +726,40 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
                 key.L[3], key.ps, key.pg, map & 0xff, map >> 8, occ, ms * 1e3);
       if (ms < best.ms) best = TuneVal{map, occ, ms};
     }
+  }
+  // Batches (FP32/FP16 fields with several rhs): the best one-rhs traversal
+  // against two rhs per lane, timed over the first min(nrhs, 16) rhs.
+  const int cnt = std::min(std::min(in->nrhs, out->nrhs), 16);
+  if (cnt >= 2 && in->prec != QCD_HIP_F64) {
+    const Batch bt{nullptr, cnt, 1, 1, 1};
+    float bbest = 1e30f;
+    auto time_batch = [&](int map, int occ, float *ms) -> int {
+      for (int w = 0; w < 2; w++)
+        QH_TRY(launch_cfg(EPI_D, false, map, occ, out, 0, in, 0, g, nullptr, 0, 0.0, 0.0, nullptr, st, nullptr, bt));
+      QH_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; r++)
+        QH_TRY(launch_cfg(EPI_D, false, map, occ, out, 0, in, 0, g, nullptr, 0, 0.0, 0.0, nullptr, st, nullptr, bt));
+      QH_CHECK(hipEventRecord(e1, st));
+      QH_CHECK(hipEventSynchronize(e1));
+      QH_CHECK(hipEventElapsedTime(ms, e0, e1));
+      *ms /= reps;
+      return 0;
+    };
+    QH_TRY(time_batch(best.map, best.occ, &bbest));
+    const float one = bbest;
+    for (int z : {2, 3, 4, 6, 8})
+      for (int occ : {0, 16}) {
+        float ms = 0.f;
+        QH_TRY(time_batch(MAP_ZMARCH2 | (z << 8), occ, &ms));
+        if (getenv("QCD_HIP_TUNE_VERBOSE"))
+          fprintf(stderr, "tune L=%dx%dx%dx%d ps=%d pg=%d %d rhs two-rhs lanes zlen=%d occ=%d: %.2f us (one-rhs %.2f)\n",
+                  key.L[0], key.L[1], key.L[2], key.L[3], key.ps, key.pg, cnt, z, occ, ms * 1e3, one * 1e3);
+        if (ms < bbest) {
+          bbest = ms;
+          best.bmap = MAP_ZMARCH2 | (z << 8);
+          best.bocc = occ;
+        }
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -649,7 +825,8 @@ int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs
   if (count < 1) return 0;
   TuneVal t;
   const bool have = tune_get(key_of(in, g), &t);
-  const int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
+  const bool two = have && count >= 2 && t.bmap != 0;
+  const int map = two ? t.bmap : have ? t.map : kDefaultMap, occ = two ? t.bocc : have ? t.occ : 0;
   const Batch bt{ids, count, in_rs, out_rs, b_rs};
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out, bt);
 }
@@ -748,12 +925,15 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
 
 // Force a configuration for these fields' key: map = 0 contiguous per XCD,
 // 1 y-slab per XCD, 2 | (n << 8) z-march of n steps, 3 | (n << 8) z-march of
-// n steps with LDS-staged t and x neighbours; occ = 0 compiler default, 2 or 3 minimum
-// waves per SIMD, 16 scheduling fence between hop directions.
+// n steps with LDS-staged t and x neighbours, 4 | (n << 8) z-march with the
+// forward z neighbour carried in LDS, 5 | (n << 8) z-march with two rhs per
+// lane (batches of FP32/FP16 D or Q; otherwise the one-rhs z-march); occ = 0
+// compiler default, 2 or 3 minimum waves per SIMD, 16 scheduling fence
+// between hop directions.
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ) {
   QH_TRY(ensure_init());
   const int m0 = map & 0xff;
-  if (!in || !gauge || m0 < 0 || m0 > 4 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
+  if (!in || !gauge || m0 < 0 || m0 > 5 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
     set_error("dslash_set_config: bad arguments");
     return 1;
   }
@@ -764,7 +944,7 @@ int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, in
       e.second = TuneVal{map, occ, 0.f};
       return 0;
     }
-  g_tune.push_back({k, TuneVal{map, occ, 0.f}});
+  g_tune.push_back({k, TuneVal{map, occ, 0.f}});   // forced: batches use the same traversal
   return 0;
 }
 

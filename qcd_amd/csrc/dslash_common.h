@@ -228,7 +228,10 @@ struct GaugeIn {
 //              loaded for one step's z hop, in an LDS slot of their own and
 //              read it back as the next step's centre spinor instead of
 //              loading it again through L2.
-enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3, MAP_ZMARCH_LDS = 4 };
+//  MAP_ZMARCH2: MAP_ZMARCH for batches (FP32 and FP16): each lane works on
+//              two right-hand sides of its column and loads every link once
+//              for both.
+enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3, MAP_ZMARCH_LDS = 4, MAP_ZMARCH2 = 5 };
 
 struct DslashArgs {
   unsigned V;
@@ -249,6 +252,7 @@ struct DslashArgs {
   unsigned in_rs, out_rs, b_rs;
   unsigned count;                  // batch rows (z-march: interleaved into grid.x, else grid.y)
   unsigned nblk;                   // workgroups per batch row
+  unsigned pairs;                  // MAP_ZMARCH2: lanes carry batch rows 2p and 2p+1; pairs = ceil(count / 2)
 };
 
 template <int PS>

@@ -184,3 +184,34 @@ def test_launch_configs_bit_identical(gpu):
                     assert rel_l2(o, ref) <= tol
         mp, occ, ms = Q.dslash_tune(y, x, g)
         assert (mp & 0xff) in (0, 1, 2, 3) and occ in (0, 2, 3, 16) and ms > 0
+
+
+def test_two_rhs_lanes_match_one_rhs_lanes(gpu):
+    """Batched FP32/FP16 launches with two right-hand sides per lane (one link
+    load for both) give the same bits as one rhs per lane, for an odd batch
+    (the last lane carries a single rhs) and for Q and D; FP64 batches and
+    single-rhs launches fall back to the one-rhs z-march."""
+    Q = gpu
+    L = (16, 8, 8, 8)
+    m = 1 / (2 * 0.13) - 4
+    u, _ = _fields(L, 0.3, 81, 82)
+    lat = Q.Lattice(L)
+    g64 = Q.Gauge(lat).upload(u)
+    for prec in ("f64", "f32", "f16"):
+        p = Q.PREC[prec]
+        g = g64
+        if p != Q.F64:
+            g = Q.Gauge(lat, p)
+            Q.convert(g, g64)
+        x = Q.Spinor(lat, p, nrhs=5)
+        for r in range(5):
+            x.upload(Q.spinor_gaussian(L, 90 + r), rhs=r)
+        y = Q.Spinor(lat, p, nrhs=5)
+        for op in (Q.OP_Q, Q.OP_D):
+            outs = []
+            for map_ in (2 | (3 << 8), 5 | (3 << 8)):
+                Q.dslash_set_config(x, g, map_, 16)
+                Q.dslash(y, x, g, m, 0.02, op)
+                outs.append([y.download(r) for r in range(5)])
+            for a, b in zip(*outs):
+                assert np.array_equal(a, b), prec
