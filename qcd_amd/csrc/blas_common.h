@@ -26,15 +26,20 @@ struct SpinorSet {
   __device__ __forceinline__ void load(int j, unsigned s, Cx<R> v[12]) const {
     const T *b = base + (size_t)j * vstride;
     const unsigned off = SpinorIn<P>::offset(s);
-#pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
     if constexpr (P == QCD_HIP_F16) {
       const R n = norm[(size_t)j * nstride + s];
 #pragma unroll
-      for (int k = 0; k < 12; k++) {
-        v[k].re *= n;
-        v[k].im *= n;
+      for (int g = 0; g < 3; g++) {
+        const u32x4 q = *(const u32x4 *)((const char *)b + g * 4 * kTile * 4 + off);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const Cx<R> c = to_cx<R>(as_half2(q[i]));
+          v[4 * g + i] = Cx<R>{c.re * n, c.im * n};
+        }
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
     }
   }
   __device__ __forceinline__ void store(int j, unsigned s, const Cx<R> v[12]) const {
@@ -50,7 +55,13 @@ struct SpinorSet {
       const R inv = mx > R(0) ? R(1) / mx : R(0);
       norm[(size_t)j * nstride + s] = mx;
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
+      for (int g = 0; g < 3; g++) {
+        u32x4 q;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
+        *(u32x4 *)((char *)b + g * 4 * kTile * 4 + off) = q;
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(v[k]));

@@ -71,6 +71,16 @@ __device__ __forceinline__ void st_plane(T *base, unsigned off, T v) {
   *(T *)((char *)base + off) = v;
 }
 
+// FP16 fields are grouped (tile_index_e): a spinor site's 12 components are
+// 3 groups of 4 (16 bytes each, 1024 bytes apart), a link's 9 are 3 groups
+// of 3 (12 bytes each, 768 bytes apart).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct alignas(4) U32x3 {
+  unsigned x, y, z;
+};
+__device__ __forceinline__ __half2 as_half2(unsigned v) { return __builtin_bit_cast(__half2, v); }
+__device__ __forceinline__ unsigned as_u32(__half2 v) { return __builtin_bit_cast(unsigned, v); }
+
 // components per base pointer for complex type T, and bases for n components
 template <typename T>
 constexpr int comp_per_base() { return 4096 / (kTile * (int)sizeof(T)); }
@@ -105,16 +115,27 @@ struct SpinorIn {
   }
   // component k of tile 0
   __device__ __forceinline__ const T *comp(int k) const { return p[k / kPer] + (k % kPer) * kTile; }
-  // byte offset of site s from its tile-0 component pointer
+  // byte offset of site s from its tile-0 component pointer (FP16: from the
+  // tile-0 pointer of its first component group)
   static __device__ __forceinline__ unsigned offset(unsigned s) {
+    if constexpr (P == QCD_HIP_F16) return ((s / kTile) * (12u * kTile) + (s % kTile) * 4u) * 4u;
     return ((s / kTile) * (12u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
   }
   // stored values of site s (and its FP16 norm)
   __device__ __forceinline__ void load_raw(unsigned s, T r[12], float &n) const {
     const unsigned off = offset(s);
+    if constexpr (P == QCD_HIP_F16) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) r[k] = ld_plane(comp(k), off);
-    if constexpr (P == QCD_HIP_F16) n = ld_plane(norm, s * 4u);
+      for (int g = 0; g < 3; g++) {
+        const u32x4 q = *(const u32x4 *)((const char *)p[0] + g * 4 * kTile * 4 + off);
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[4 * g + i] = as_half2(q[i]);
+      }
+      n = ld_plane(norm, s * 4u);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) r[k] = ld_plane(comp(k), off);
+    }
   }
   static __device__ __forceinline__ void convert(const T r[12], float n, Cx<R> v[12]) {
 #pragma unroll
@@ -173,7 +194,13 @@ struct SpinorOut {
       const R inv = mx > R(0) ? R(1) / mx : R(0);
       st_plane(norm, s * 4u, (float)mx);
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(Cx<R>{v[k].re * inv, v[k].im * inv}));
+      for (int g = 0; g < 3; g++) {
+        u32x4 q;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
+        *(u32x4 *)((char *)p[0] + g * 4 * kTile * 4 + off) = q;
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(v[k]));
@@ -196,9 +223,20 @@ struct GaugeIn {
   __device__ __forceinline__ const T *comp(int mu, int k) const { return p[kNb * mu + k / kPer] + (k % kPer) * kTile; }
   // stored values of the link of direction mu at site s
   __device__ __forceinline__ void load_raw(int mu, unsigned s, T r[9]) const {
-    const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+    if constexpr (P == QCD_HIP_F16) {
+      const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile) * 3u) * 4u;
 #pragma unroll
-    for (int k = 0; k < 9; k++) r[k] = ld_plane(comp(mu, k), off);
+      for (int g = 0; g < 3; g++) {
+        const U32x3 q = *(const U32x3 *)((const char *)p[kNb * mu] + g * 3 * kTile * 4 + off);
+        r[3 * g] = as_half2(q.x);
+        r[3 * g + 1] = as_half2(q.y);
+        r[3 * g + 2] = as_half2(q.z);
+      }
+    } else {
+      const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+#pragma unroll
+      for (int k = 0; k < 9; k++) r[k] = ld_plane(comp(mu, k), off);
+    }
   }
   template <typename R>
   __device__ __forceinline__ void load(int mu, unsigned s, Cx<R> u[9]) const {

@@ -110,6 +110,24 @@ __host__ __device__ __forceinline__ size_t tile_index(int nc, int k, size_t s) {
 __host__ __device__ __forceinline__ size_t gauge_index(int k, size_t s, size_t stride) {
   return (size_t)(k / 9) * 9 * stride + tile_index(9, k % 9, s);
 }
+// FP16 fields (4-byte elements) group their components inside a tile: the
+// G = 4 (spinor, nc = 12) or G = 3 (link, nc = 9) consecutive components of
+// a site are adjacent, so a lane reads a group with one 16- or 12-byte load:
+// tile [nc/G groups][64 sites][G].  Wider elements keep one plane per
+// component.  ESZ is the element size in bytes.
+template <int ESZ>
+__host__ __device__ __forceinline__ size_t tile_index_e(int nc, int k, size_t s) {
+  if constexpr (ESZ == 4) {
+    const int G = nc % 4 == 0 ? 4 : 3;
+    return ((s / kTile) * (size_t)nc + (size_t)(k / G) * G) * kTile + (s % kTile) * G + (size_t)(k % G);
+  } else {
+    return tile_index(nc, k, s);
+  }
+}
+template <int ESZ>
+__host__ __device__ __forceinline__ size_t gauge_index_e(int k, size_t s, size_t stride) {
+  return (size_t)(k / 9) * 9 * stride + tile_index_e<ESZ>(9, k % 9, s);
+}
 
 // Hardware dispatches workgroup i to XCD i % 8.  Remap so each XCD owns one
 // contiguous range of logical blocks: neighbouring sites then share an L2.

@@ -158,9 +158,12 @@ int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_fi
 }
 
 // ------------------------------------------------------------ layout kernels
-// index of component k of site s: NC = 12 spinor, NC = 36 gauge field
+// index of component k of site s in a field of element type T: NC = 12
+// spinor, NC = 36 gauge field
+template <typename T>
 __device__ __forceinline__ size_t elem(int nc, int k, long s, long stride) {
-  return nc == 36 ? gauge_index(k, (size_t)s, (size_t)stride) : tile_index(nc, k, (size_t)s);
+  return nc == 36 ? gauge_index_e<(int)sizeof(T)>(k, (size_t)s, (size_t)stride)
+                  : tile_index_e<(int)sizeof(T)>(nc, k, (size_t)s);
 }
 
 
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_planes(const double2 *__restr
   long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= V) return;
 #pragma unroll
-  for (int k = 0; k < NC; k++) planes[elem(NC, k, s, stride)] = cvt2<Dst>(aos[s * NC + k]);
+  for (int k = 0; k < NC; k++) planes[elem<Dst>(NC, k, s, stride)] = cvt2<Dst>(aos[s * NC + k]);
 }
 
 template <typename Src, int NC>
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_planes_to_aos(const Src *__restrict_
   if (s >= V) return;
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    Src v = planes[elem(NC, k, s, stride)];
+    Src v = planes[elem<Src>(NC, k, s, stride)];
     double2 o;
     if constexpr (sizeof(Src) == 16) {
       o = *(const double2 *)&v;
@@ -222,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_half_spinor(const double2 *__
   float inv = n > 0.f ? 1.0f / n : 0.f;
   norm[s] = n;
 #pragma unroll
-  for (int k = 0; k < 12; k++) planes[tile_index(12, k, s)] = __floats2half2_rn((float)v[k].x * inv, (float)v[k].y * inv);
+  for (int k = 0; k < 12; k++) planes[tile_index_e<4>(12, k, s)] = __floats2half2_rn((float)v[k].x * inv, (float)v[k].y * inv);
 }
 
 __global__ void __launch_bounds__(kBlock) k_half_spinor_to_aos(const __half2 *__restrict__ planes, const float *__restrict__ norm,
@@ -232,7 +235,7 @@ __global__ void __launch_bounds__(kBlock) k_half_spinor_to_aos(const __half2 *__
   float n = norm[s];
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    float2 f = __half22float2(planes[tile_index(12, k, s)]);
+    float2 f = __half22float2(planes[tile_index_e<4>(12, k, s)]);
     aos[s * 12 + k] = make_double2((double)f.x * n, (double)f.y * n);
   }
 }
@@ -262,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict
   double mx = 0.0;
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    v[k] = load_as_double(src, elem(NC, k, s, stride), snorm, s);
+    v[k] = load_as_double(src, elem<Src>(NC, k, s, stride), snorm, s);
     mx = fmax(mx, fmax(fabs(v[k].x), fabs(v[k].y)));
   }
   double inv = 1.0;
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_convert_planes(const Src *__restrict
     inv = n > 0.f ? 1.0 / (double)n : 0.0;
   }
 #pragma unroll
-  for (int k = 0; k < NC; k++) dst[elem(NC, k, s, stride)] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
+  for (int k = 0; k < NC; k++) dst[elem<Dst>(NC, k, s, stride)] = cvt2<Dst>(make_double2(v[k].x * inv, v[k].y * inv));
 }
 
 }  // namespace qcdhip
