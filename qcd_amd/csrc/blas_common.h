@@ -37,6 +37,13 @@ struct SpinorSet {
           v[4 * g + i] = Cx<R>{c.re * n, c.im * n};
         }
       }
+    } else if constexpr (P == QCD_HIP_F32) {
+#pragma unroll
+      for (int g = 0; g < 6; g++) {
+        const float4 q = *(const float4 *)((const char *)b + g * 2 * kTile * 8 + off);
+        v[2 * g] = Cx<R>{q.x, q.y};
+        v[2 * g + 1] = Cx<R>{q.z, q.w};
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
@@ -62,6 +69,11 @@ struct SpinorSet {
           q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
         *(u32x4 *)((char *)b + g * 4 * kTile * 4 + off) = q;
       }
+    } else if constexpr (P == QCD_HIP_F32) {
+#pragma unroll
+      for (int g = 0; g < 6; g++)
+        *(float4 *)((char *)b + g * 2 * kTile * 8 + off) =
+            make_float4((float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im);
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(v[k]));

@@ -115,11 +115,12 @@ struct SpinorIn {
   }
   // component k of tile 0
   __device__ __forceinline__ const T *comp(int k) const { return p[k / kPer] + (k % kPer) * kTile; }
-  // byte offset of site s from its tile-0 component pointer (FP16: from the
-  // tile-0 pointer of its first component group)
+  // components per 16-byte group (tile_index_e): 4 (FP16), 2 (FP32), 1 (FP64)
+  static constexpr int kG = 16 / (int)sizeof(T);
+  // byte offset of site s from the tile-0 pointer of its first component
+  // (group)
   static __device__ __forceinline__ unsigned offset(unsigned s) {
-    if constexpr (P == QCD_HIP_F16) return ((s / kTile) * (12u * kTile) + (s % kTile) * 4u) * 4u;
-    return ((s / kTile) * (12u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
+    return ((s / kTile) * (12u * kTile) + (s % kTile) * (unsigned)kG) * (unsigned)sizeof(T);
   }
   // stored values of site s (and its FP16 norm)
   __device__ __forceinline__ void load_raw(unsigned s, T r[12], float &n) const {
@@ -132,6 +133,15 @@ struct SpinorIn {
         for (int i = 0; i < 4; i++) r[4 * g + i] = as_half2(q[i]);
       }
       n = ld_plane(norm, s * 4u);
+    } else if constexpr (P == QCD_HIP_F32) {
+#pragma unroll
+      for (int g = 0; g < 6; g++) {
+        // groups 4 and 5 start at the second base (element 8 * 64), so
+        // every immediate offset stays below 4096 bytes
+        const float4 q = *(const float4 *)((const char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off);
+        r[2 * g] = make_float2(q.x, q.y);
+        r[2 * g + 1] = make_float2(q.z, q.w);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) r[k] = ld_plane(comp(k), off);
@@ -201,6 +211,11 @@ struct SpinorOut {
           q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
         *(u32x4 *)((char *)p[0] + g * 4 * kTile * 4 + off) = q;
       }
+    } else if constexpr (P == QCD_HIP_F32) {
+#pragma unroll
+      for (int g = 0; g < 6; g++)
+        *(float4 *)((char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off) =
+            make_float4((float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im);
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(v[k]));

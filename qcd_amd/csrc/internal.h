@@ -110,19 +110,20 @@ __host__ __device__ __forceinline__ size_t tile_index(int nc, int k, size_t s) {
 __host__ __device__ __forceinline__ size_t gauge_index(int k, size_t s, size_t stride) {
   return (size_t)(k / 9) * 9 * stride + tile_index(9, k % 9, s);
 }
-// FP16 fields (4-byte elements) group their components inside a tile: the
-// G = 4 (spinor, nc = 12) or G = 3 (link, nc = 9) consecutive components of
-// a site are adjacent, so a lane reads a group with one 16- or 12-byte load:
-// tile [nc/G groups][64 sites][G].  Wider elements keep one plane per
-// component.  ESZ is the element size in bytes.
+// Reduced-precision fields group their components inside a tile: G
+// consecutive components of a site are adjacent, so a lane reads a group
+// with one 16-byte (or, for FP16 links, 12-byte) load: tile
+// [nc/G groups][64 sites][G].  G = 4 for FP16 spinors (nc = 12), 3 for FP16
+// links (nc = 9), 2 for FP32 spinors; FP32 links and FP64 fields keep one
+// plane per component (G = 1).  ESZ is the element size in bytes.
+__host__ __device__ constexpr int tile_group(int esz, int nc) {
+  return esz == 4 ? (nc == 12 ? 4 : 3) : (esz == 8 && nc == 12) ? 2 : 1;
+}
 template <int ESZ>
 __host__ __device__ __forceinline__ size_t tile_index_e(int nc, int k, size_t s) {
-  if constexpr (ESZ == 4) {
-    const int G = nc % 4 == 0 ? 4 : 3;
-    return ((s / kTile) * (size_t)nc + (size_t)(k / G) * G) * kTile + (s % kTile) * G + (size_t)(k % G);
-  } else {
-    return tile_index(nc, k, s);
-  }
+  const int G = tile_group(ESZ, nc);
+  if (G == 1) return tile_index(nc, k, s);
+  return ((s / kTile) * (size_t)nc + (size_t)(k / G) * G) * kTile + (s % kTile) * G + (size_t)(k % G);
 }
 template <int ESZ>
 __host__ __device__ __forceinline__ size_t gauge_index_e(int k, size_t s, size_t stride) {
