@@ -170,5 +170,7 @@ void qcd_waitall(qcd_geometry *geo);
 #include <qcd_smearing.h>
 #include <qcd_io.h>
 #include <qcd_observables.h>
+#include <qcd_rng.h>
+#include <qcd_stochastic.h>
 
 #endif

@@ -73,6 +73,12 @@ int qcd_hip_gauge_generate(qcd_hip_field gauge64, double eps, uint64_t seed);
 /* rhs of an FP64 spinor field = delta at site {t,x,y,z} on (spin, col)
  * (source.c:203-208) */
 int qcd_hip_point_source(qcd_hip_field f, int rhs, const int site[4], int spin, int col);
+/* Z4 noise ({+1, +i, -1, -i}) in vector rhs of an FP64 spinor on time slice
+ * t_slice (all slices when t_slice < 0), zero elsewhere, on the device.  The
+ * value of (site, spin, colour) is a counter-based draw under seed, so it
+ * does not depend on launch order; the stream differs from the host one of
+ * qcd_z2Vector (lib/src/qcd_stochastic.c:8-58). */
+int qcd_hip_z4_source(qcd_hip_field f, int rhs, int t_slice, uint64_t seed);
 
 /* ------------------------------------------------------------ operators
  * out = D in (op = QCD_HIP_OP_D) or Q in = i gamma5 D in (QCD_HIP_OP_Q),

@@ -142,6 +142,11 @@ def lib():
         "qcd_hip_b_minus_Dx": (i, [_vp, i, _vp, i, _vp, d, d, _dp]),
         "qcd_hip_gauge_generate": (i, [_vp, d, ctypes.c_uint64]),
         "qcd_hip_point_source": (i, [_vp, i, ctypes.POINTER(ctypes.c_int), i, i]),
+        "qcd_hip_z4_source": (i, [_vp, i, i, ctypes.c_uint64]),
+        "qcd_rng_init": (None, [ctypes.c_uint, i]),
+        "qcd_rng_finalize": (None, []),
+        "qcd_get_rand_4": (ctypes.c_ulong, []),
+        "qcd_z2Vector": (None, [pf, i]),
         "qcd_hip_release_cache": (None, []),
         "qcd_hip_serve": (i, [ctypes.POINTER(Request), i, i, sz, i, ctypes.POINTER(Result), _dp,
                               ctypes.POINTER(_dp)]),
@@ -312,6 +317,11 @@ def gauge_generate(L, eps, seed):
 def gauge_generate_device(gauge64, eps, seed):
     """The generator recipe run on the GPU into an FP64 gauge field."""
     _check(lib().qcd_hip_gauge_generate(gauge64.h, float(eps), ctypes.c_uint64(seed)), "qcd_hip_gauge_generate")
+
+
+def z4_source(f, rhs, t_slice, seed):
+    """rhs of an FP64 device spinor = Z4 noise on time slice t_slice (all slices when negative)."""
+    _check(lib().qcd_hip_z4_source(f.h, int(rhs), int(t_slice), int(seed)), "qcd_hip_z4_source")
 
 
 def point_source_device(f, rhs, site, spin, col):
