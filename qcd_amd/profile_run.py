@@ -7,7 +7,7 @@ the same lattice, so that the counters of the copy calibrate those of Dslash.
 
        python -m qcd_amd.profile_run --solve --lattice 48 24 24 24 --prec f16 --rhs 48 --eo 1
 Runs one propagator solve instead: 12 point sources (spin x colour) at each
-of rhs/12 sites, kappa 0.132, eps 0.3, GCR(32), tol 1e-10, all rhs in one
+of rhs/12 sites, kappa 0.132, eps 0.3, GCR(n_restart), tol 1e-10, all rhs in one
 batch, so that a kernel trace shows where a solve spends its time.
 """
 import argparse
@@ -32,11 +32,12 @@ def main(argv=None):
     ap.add_argument("--solve", action="store_true", help="profile one batched propagator solve instead")
     ap.add_argument("--rhs", type=int, default=12, help="--solve: right-hand sides (a multiple of 12)")
     ap.add_argument("--eo", type=int, default=1, help="--solve: even-odd preconditioning")
+    ap.add_argument("--n-restart", type=int, default=32, help="--solve: GCR cycle length (invert.ini: 32)")
     args = ap.parse_args(argv)
     L = tuple(args.lattice)
     Q.init(0)
     if args.solve:
-        return solve(Q, L, args.prec, args.rhs, args.eo)
+        return solve(Q, L, args.prec, args.rhs, args.eo, args.n_restart)
     lat = Q.Lattice(L)
     u = Q.gauge_generate(L, 0.3, 4001)
     psi = Q.spinor_gaussian(L, 4002)
@@ -66,7 +67,7 @@ def main(argv=None):
     return 0
 
 
-def solve(Q, L, prec, nrhs, eo):
+def solve(Q, L, prec, nrhs, eo, n_restart=32):
     lat = Q.Lattice(L)
     g = Q.Gauge(lat)
     Q.gauge_generate_device(g, 0.3, 3003)
@@ -77,7 +78,8 @@ def solve(Q, L, prec, nrhs, eo):
         for r in range(12):
             Q.point_source_device(b, 12 * i + r, site, r // 3, r % 3)
     x = Q.Spinor(lat, nrhs=12 * len(sites))
-    S = Q.Solver(lat, g, n_restart=32, tol=1e-10, inner_prec=prec, batch=12 * len(sites), eo=eo, max_outer=20000)
+    S = Q.Solver(lat, g, n_restart=n_restart, tol=1e-10, inner_prec=prec, batch=12 * len(sites), eo=eo,
+                 max_outer=20000)
     st = S.solve(x, b, 0.132)
     Q.synchronize()
     print(f"lattice {L} prec={prec} rhs={12 * len(sites)} eo={eo}: {st['seconds']:.3f} s, "
