@@ -471,8 +471,7 @@ __device__ __forceinline__ Cx<R> diag_term(Cx<R> v, R mp4, R gtm) {
 
 // the colour-multiplied, phased half spinor g of one hop (before rebuilding)
 template <int MU, int S, bool DAG, typename R>
-__device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> psi[12], const Cx<R> u[9], bool twisted,
-                                         Cx<R> ph) {
+__device__ __forceinline__ void hop_half(Cx<R> g0[3], Cx<R> g1[3], const Cx<R> psi[12], const Cx<R> u[9], Cx<R> ph) {
   using PJ = Proj<MU>;
   constexpr int sp = S > 0 ? 0 : 2;
   Cx<R> h0[3], h1[3];
@@ -615,7 +614,7 @@ __device__ __forceinline__ void hop_half_raw(Cx<R> g0[3], Cx<R> g1[3], const typ
     SpinorIn<PS>::convert(r, n, psi);
 #pragma unroll
     for (int k = 0; k < 9; k++) uc[k] = to_cx<R>(u[k]);
-    hop_half<MU, S, DAG>(g0, g1, psi, uc, false, ph);
+    hop_half<MU, S, DAG>(g0, g1, psi, uc, ph);
   }
 }
 
