@@ -31,17 +31,17 @@ struct EoArgs {
 // the group follow, so the z-neighbour planes read a few site blocks later are
 // still in that L2.  rgroup = count shares the links most, rgroup = 1 keeps
 // one row's spinors most local.
-__device__ __forceinline__ BlockRow eo_row(const EoArgs &e) {
+__device__ __forceinline__ BlockRow eo_row_of(unsigned count, unsigned R, unsigned nb) {
   const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  const unsigned R = e.rgroup, nb = e.d.nblk;
-  const unsigned full = e.d.count / R, span = nb * R;
+  const unsigned full = count / R, span = nb * R;
   if (lb < full * span) {
     const unsigned g = lb / span, w = lb - g * span;
     return BlockRow{g * R + w % R, w / R};
   }
-  const unsigned rem = e.d.count - full * R, w = lb - full * span;
+  const unsigned rem = count - full * R, w = lb - full * span;
   return BlockRow{full * R + w % rem, w / rem};
 }
+__device__ __forceinline__ BlockRow eo_row(const EoArgs &e) { return eo_row_of(e.d.count, e.rgroup, e.d.nblk); }
 
 // the site of parity par stored at half-lattice entry h
 __device__ __forceinline__ Site eo_site(unsigned h, unsigned par, const DslashArgs &a) {
@@ -98,6 +98,102 @@ __device__ __forceinline__ void eo_order_point(Cx<R> acc[4][3]) {
   }
 }
 
+// o from H in = -1/2 acc by the epilogue EPI (b read at half-site h)
+template <int EPI, int PS, typename R>
+__device__ __forceinline__ void eo_epilogue(Cx<R> o[12], const Cx<R> acc[4][3], const EoArgs &e,
+                                            const SpinorIn<PS> &b, unsigned h) {
+  if constexpr (EPI == EO_MINV) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) o[3 * q + c] = Cx<R>{R(-0.5) * acc[q][c].re, R(-0.5) * acc[q][c].im};
+    eo_mass<true>(o, e);
+  } else {
+    Cx<R> bv[12];
+    b.load(h, bv);
+    if constexpr (EPI == EO_SCHUR) eo_mass<false>(bv, e);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        o[3 * q + c] = Cx<R>{fma(R(0.5), acc[q][c].re, bv[3 * q + c].re), fma(R(0.5), acc[q][c].im, bv[3 * q + c].im)};
+    if constexpr (EPI == EO_SUB_MINV) eo_mass<true>(o, e);
+  }
+}
+
+// both hops of direction MU for two right-hand sides: links loaded once
+template <int MU, int PS, int PG, typename R>
+__device__ __forceinline__ void eo_hop_dir2(Cx<R> acc0[4][3], Cx<R> acc1[4][3], const DslashArgs &a, const Site &st,
+                                            const SpinorIn<PS> &in0, const SpinorIn<PS> &in1, const GaugeIn<PG> &U) {
+  const Cx<R> php{(R)a.ph_re[MU], (R)a.ph_im[MU]};
+  const Cx<R> phm{(R)a.ph_re[MU], -(R)a.ph_im[MU]};
+  typename Prec<PS>::T r[12];
+  typename Prec<PG>::T u[9];
+  Cx<R> g0[3], g1[3];
+  float n = 0.f;
+  {
+    const unsigned sf = neighbour<MU, +1>(st, a) >> 1;
+    U.load_raw(MU, st.s, u);
+    in0.load_raw(sf, r, n);
+    hop_half_raw<MU, -1, false, PS, PG>(g0, g1, r, n, u, php);
+    rebuild_add<MU, -1>(acc0, g0, g1);
+    in1.load_raw(sf, r, n);
+    hop_half_raw<MU, -1, false, PS, PG>(g0, g1, r, n, u, php);
+    rebuild_add<MU, -1>(acc1, g0, g1);
+  }
+  {
+    const unsigned sb = neighbour<MU, -1>(st, a);
+    U.load_raw(MU, sb, u);
+    in0.load_raw(sb >> 1, r, n);
+    hop_half_raw<MU, +1, true, PS, PG>(g0, g1, r, n, u, phm);
+    rebuild_add<MU, +1>(acc0, g0, g1);
+    in1.load_raw(sb >> 1, r, n);
+    hop_half_raw<MU, +1, true, PS, PG>(g0, g1, r, n, u, phm);
+    rebuild_add<MU, +1>(acc1, g0, g1);
+  }
+}
+
+// k_eo_hop with two right-hand sides per lane (batch rows 2p and 2p+1 of pair
+// p; an odd batch's last pair repeats its first row and stores it once), for
+// FP32/FP16 batches with the M^-1 H or M b - H epilogue.  Per rhs the same
+// arithmetic in the same order as k_eo_hop: bit-identical results.
+template <int PS, int PG, int EPI>
+__global__ void __launch_bounds__(kBlock) k_eo_hop2(EoArgs e, SpinorIn<PS> in_all, GaugeIn<PG> U,
+                                                    SpinorOut<PS> out_all, SpinorIn<PS> b_all) {
+  using R = typename Prec<PS>::R;
+  const DslashArgs &a = e.d;
+  const unsigned pairs = (a.count + 1) / 2;
+  const BlockRow br = eo_row_of(pairs, max(1u, e.rgroup / 2), a.nblk);
+  const unsigned r0 = 2 * br.row, r1 = min(2 * br.row + 1, a.count - 1);
+  const RhsView<PS> v0 = rhs_view(a, in_all, out_all, b_all, r0);
+  const RhsView<PS> v1 = rhs_view(a, in_all, out_all, b_all, r1);
+  const unsigned h = br.blk * kBlock + threadIdx.x;
+  if (h >= e.Vh) return;
+  const Site st = eo_site(h, e.par, a);
+  Cx<R> acc0[4][3], acc1[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc0[q][c] = acc1[q][c] = Cx<R>{R(0), R(0)};
+  eo_hop_dir2<0, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+  eo_order_point<PS>(acc0);
+  eo_order_point<PS>(acc1);
+  eo_hop_dir2<1, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+  eo_order_point<PS>(acc0);
+  eo_order_point<PS>(acc1);
+  eo_hop_dir2<2, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+  eo_order_point<PS>(acc0);
+  eo_order_point<PS>(acc1);
+  eo_hop_dir2<3, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
+  Cx<R> o[12];
+  eo_epilogue<EPI>(o, acc0, e, v0.b, h);
+  v0.out.store(h, o);
+  if (r1 != r0) {
+    eo_epilogue<EPI>(o, acc1, e, v1.b, h);
+    v1.out.store(h, o);
+  }
+}
+
 // one lane per output site; workgroups map to (batch row, site block) by eo_row
 template <int PS, int PG, int EPI, bool RED>
 __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all, GaugeIn<PG> U,
@@ -128,25 +224,8 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all
     eo_hop_dir<2, PS, PG, R>(acc, a, st, rv.in, U);
     eo_order_point<PS>(acc);
     eo_hop_dir<3, PS, PG, R>(acc, a, st, rv.in, U);
-    // H in = -1/2 acc
     Cx<R> o[12];
-    if constexpr (EPI == EO_MINV) {
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) o[3 * q + c] = Cx<R>{R(-0.5) * acc[q][c].re, R(-0.5) * acc[q][c].im};
-      eo_mass<true>(o, e);
-    } else {
-      Cx<R> bv[12];
-      rv.b.load(h, bv);
-      if constexpr (EPI == EO_SCHUR) eo_mass<false>(bv, e);
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-          o[3 * q + c] = Cx<R>{fma(R(0.5), acc[q][c].re, bv[3 * q + c].re), fma(R(0.5), acc[q][c].im, bv[3 * q + c].im)};
-      if constexpr (EPI == EO_SUB_MINV) eo_mass<true>(o, e);
-    }
+    eo_epilogue<EPI>(o, acc, e, rv.b, h);
     if (RED) {
 #pragma unroll
       for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
@@ -258,6 +337,16 @@ qcd_hip_lattice eo_half_lattice(qcd_hip_lattice full) {
   return h;
 }
 
+// FP32/FP16 batches of the M^-1 H and M b - H hops run two rhs per lane
+// (k_eo_hop2) unless QCD_HIP_EO_TWO=0.
+static bool eo_two_rhs() {
+  static const bool on = [] {
+    const char *s = getenv("QCD_HIP_EO_TWO");
+    return !(s && s[0] == '0');
+  }();
+  return on;
+}
+
 template <int PS, int PG>
 static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
                         int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st) {
@@ -266,6 +355,15 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
   const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
   const GaugeIn<PG> gu = GaugeIn<PG>::of(g);
 #define QH_E(E, RD) k_eo_hop<PS, PG, E, RD><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
+  if constexpr (PS != QCD_HIP_F64) {
+    if (e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) && eo_two_rhs()) {
+      const dim3 g2(e.d.nblk * ((e.d.count + 1) / 2));
+      if (epi == EO_MINV) k_eo_hop2<PS, PG, EO_MINV><<<g2, kBlock, 0, st>>>(e, si, gu, so, sb);
+      else k_eo_hop2<PS, PG, EO_SCHUR><<<g2, kBlock, 0, st>>>(e, si, gu, so, sb);
+      QH_CHECK(hipGetLastError());
+      return 0;
+    }
+  }
   switch (epi * 2 + (red ? 1 : 0)) {
     case 0: QH_E(EO_MINV, false); break;
     case 2: QH_E(EO_SCHUR, false); break;
