@@ -663,6 +663,17 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
   a.b_rs = (unsigned)bt.b_rs;
   a.nblk = (unsigned)grid;
   a.pairs = (a.count + 1) / 2;
+  {
+    // batch rows in groups of 4 (block_row): measured at 24^3x48 against 1, 2,
+    // 8 and the whole batch, 4 was the best or within 2% (FP64 12/48 rhs -3%,
+    // FP32 16 rhs -10% against the whole batch).  QCD_HIP_DSLASH_RGROUP
+    // overrides it (0 = the whole batch).
+    static const long env = [] {
+      const char *e = getenv("QCD_HIP_DSLASH_RGROUP");
+      return e ? std::atol(e) : -1L;
+    }();
+    a.rgroup = env >= 0 ? (unsigned)env : 4u;
+  }
   if (a.map == MAP_ZMARCH2 && (a.count < 2 || in->prec == QCD_HIP_F64 || red || (epi != EPI_D && epi != EPI_Q))) {
     // the two-rhs traversal does not apply: same z-march, one rhs per lane
     a.map = MAP_ZMARCH;

@@ -306,6 +306,7 @@ struct DslashArgs {
   unsigned count;                  // batch rows (z-march: interleaved into grid.x, else grid.y)
   unsigned nblk;                   // workgroups per batch row
   unsigned pairs;                  // MAP_ZMARCH2: lanes carry batch rows 2p and 2p+1; pairs = ceil(count / 2)
+  unsigned rgroup;                 // z-march batches: rows per group (block_row); 0 = the whole batch
 };
 
 template <int PS>
@@ -334,7 +335,18 @@ struct BlockRow {
 };
 __device__ __forceinline__ BlockRow block_row(const DslashArgs &a) {
   const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  return BlockRow{lb % a.count, lb / a.count};
+  const unsigned R = a.rgroup;
+  if (R == 0 || R >= a.count) return BlockRow{lb % a.count, lb / a.count};
+  // rows in groups of R, the row fastest within a group: a group's site
+  // blocks share links through one L2 while enough site blocks of a group
+  // are in flight to share their neighbour planes too
+  const unsigned nb = a.nblk, full = a.count / R, span = nb * R;
+  if (lb < full * span) {
+    const unsigned g = lb / span, w = lb - g * span;
+    return BlockRow{g * R + w % R, w / R};
+  }
+  const unsigned rem = a.count - full * R, w = lb - full * span;
+  return BlockRow{full * R + w % rem, w / rem};
 }
 
 struct Site {
