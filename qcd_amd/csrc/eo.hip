@@ -236,17 +236,23 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all
 }
 
 // full FP64 vector f0 + ids[row] -> half vectors ev[slot], od[slot]
+// One lane per full-lattice site s (consecutive lanes, consecutive sites, so
+// the full vector is read and written in whole segments); its half-lattice
+// entry is s >> 1 of the field of its parity.
+__device__ __forceinline__ unsigned site_parity(unsigned s, const DslashArgs &a) {
+  const Site st = site_coords(s, a);
+  return (st.t + st.x + st.y + st.z) & 1u;
+}
+
 __global__ void __launch_bounds__(kBlock) k_eo_split(EoArgs e, SpinorSet<QCD_HIP_F64> full, int f0,
                                                      SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
                                                      const int *ids) {
   const int b = ids[blockIdx.y];
-  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
-  if (h >= e.Vh) return;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= e.d.V) return;
   Cx<double> v[12];
-  full.load(f0 + b, eo_site(h, 0, e.d).s, v);
-  ev.store(b, h, v);
-  full.load(f0 + b, eo_site(h, 1, e.d).s, v);
-  od.store(b, h, v);
+  full.load(f0 + b, s, v);
+  (site_parity(s, e.d) ? od : ev).store(b, s >> 1, v);
 }
 
 // x[f0 + slot] += (ev[slot] on even sites, od[slot] on odd sites)
@@ -254,18 +260,14 @@ __global__ void __launch_bounds__(kBlock) k_eo_accumulate(EoArgs e, SpinorSet<QC
                                                           SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
                                                           const int *ids) {
   const int b = ids[blockIdx.y];
-  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
-  if (h >= e.Vh) return;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= e.d.V) return;
+  Cx<double> xv[12], d[12];
+  x.load(f0 + b, s, xv);
+  (site_parity(s, e.d) ? od : ev).load(b, s >> 1, d);
 #pragma unroll
-  for (unsigned p = 0; p < 2; p++) {
-    const unsigned s = eo_site(h, p, e.d).s;
-    Cx<double> xv[12], d[12];
-    x.load(f0 + b, s, xv);
-    (p ? od : ev).load(b, h, d);
-#pragma unroll
-    for (int k = 0; k < 12; k++) xv[k] = Cx<double>{xv[k].re + d[k].re, xv[k].im + d[k].im};
-    x.store(f0 + b, s, xv);
-  }
+  for (int k = 0; k < 12; k++) xv[k] = Cx<double>{xv[k].re + d[k].re, xv[k].im + d[k].im};
+  x.store(f0 + b, s, xv);
 }
 
 // out[slot] = M^-1 in[slot] (FP64 half vectors)
@@ -409,7 +411,7 @@ int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_
 int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
              hipStream_t st) {
   const EoArgs e = make_eo_args(full->lat, 0.0, 0.0, 0);
-  k_eo_split<<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(full), f0,
+  k_eo_split<<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(full), f0,
                                                               SpinorSet<QCD_HIP_F64>::of(ev),
                                                               SpinorSet<QCD_HIP_F64>::of(od), ids);
   QH_CHECK(hipGetLastError());
@@ -419,7 +421,7 @@ int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, con
 int eo_accumulate(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
                   hipStream_t st) {
   const EoArgs e = make_eo_args(x->lat, 0.0, 0.0, 0);
-  k_eo_accumulate<<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(x), f0,
+  k_eo_accumulate<<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(x), f0,
                                                                    SpinorSet<QCD_HIP_F64>::of(ev),
                                                                    SpinorSet<QCD_HIP_F64>::of(od), ids);
   QH_CHECK(hipGetLastError());
