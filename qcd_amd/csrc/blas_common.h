@@ -91,9 +91,9 @@ int lincomb_launch(qcd_hip_field out, int ro, double are, double aim, qcd_hip_fi
                    qcd_hip_field y, int ry, int t0, hipStream_t st);
 // cached scratch spinor number `slot` (resized on demand), owned by the library
 int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_field *out);
-// n iterations of Gauss smearing of every rhs of v on slice t0 (all when < 0)
-int gauss_iterate(qcd_hip_field v, qcd_hip_field g, qcd_hip_field tmp, double alpha, int t0, int niter,
-                  hipStream_t st);
+// niter >= 1 iterations of Gauss smearing of every rhs of v on slice t0 (all
+// slices when t0 < 0), through the library's scratch spinor 0
+int gauss_smear_field(qcd_hip_field v, qcd_hip_field g, double alpha, int t0, int niter, hipStream_t st);
 
 // even-odd preconditioning applies (every extent even), eo.hip
 bool eo_supported(qcd_hip_lattice lat);

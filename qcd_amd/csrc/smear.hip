@@ -4,37 +4,100 @@
 //   v(x) <- ( v(x) + alpha * sum_{nu=1..3} [U_nu(x) v(x+nu) + U_nu(x-nu)^+ v(x-nu)] ) / (1 + 6 alpha)
 // The hop stays inside the time-slice, so an iteration on one slice reads and
 // writes only that slice: V/T sites instead of the reference's whole-volume
-// zero / add passes (lib/src/qcd_smearing.c:45, :102).  Iterations ping-pong
-// between the field and a slice-sized-or-larger scratch field; the scale and
-// add of the reference are fused into the hop kernel.
+// zero / add passes (lib/src/qcd_smearing.c:45, :102).  The scale and add of
+// the reference are fused into the hop kernel.
+//
+// Layout of the iterations.  Sites of one time-slice lie T apart in the field
+// (l = t + T(x + X(y + Yz))), so a 64-site tile of the field holds at most
+// ceil(64/T) of them and a wavefront's loads of one component touch as many
+// cache lines as it has lanes.  The iterations therefore run on two compact
+// buffers A and B that hold the slice in (x, y, z) order, r = x + X(y + Yz):
+//   gather v -> A;  A -> B -> A ...;  the last iteration writes v itself,
+// so the strided field is read once and written once per call whatever the
+// iteration count.  The slice's spatial links are copied once per call into a
+// compact gauge field in the same order, so every iteration reads links and
+// spinors with whole-tile loads.  Every launch covers all right-hand sides,
+// with the right-hand side the fastest block index so the rows of one block
+// of sites run together on one XCD and share its links through that XCD's L2.
+// The whole-volume form (t < 0) is the same with T slices per pass; its
+// compact order is the field's own.
 #include "blas_common.h"
 
 namespace qcdhip {
 
-template <int P>
-__global__ void __launch_bounds__(kBlock) k_gauss(DslashArgs a, SpinorIn<P> in, GaugeIn<P> U, SpinorOut<P> out,
-                                                  double alpha, double nrm, int t0, unsigned nsites) {
+struct GaussArgs {
+  unsigned T, X, Y, Z;   // lattice extents
+  unsigned tn;           // time values per pass: 1 (one slice) or T (whole volume)
+  unsigned t0;           // the slice when tn == 1
+  unsigned n;            // sites per right-hand side: tn X Y Z
+  unsigned count;        // right-hand sides in the launch
+};
+
+// lane -> (row, compact site j); j = tt + tn r with r = x + X(y + Yz)
+struct GaussSite {
+  unsigned row, tt, r, x, y, z;
+  bool valid;
+};
+
+__device__ __forceinline__ GaussSite gauss_site(const GaussArgs &g) {
+  GaussSite q;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  q.row = lb % g.count;
+  const unsigned j = (lb / g.count) * kBlock + threadIdx.x;
+  q.valid = j < g.n;
+  unsigned r = j;
+  q.tt = 0;
+  if (g.tn > 1) {
+    r = j / g.tn;
+    q.tt = j - r * g.tn;
+  }
+  q.r = r;
+  const unsigned r2 = r / g.X;
+  q.x = r - r2 * g.X;
+  q.z = r2 / g.Y;
+  q.y = r2 - q.z * g.Y;
+  return q;
+}
+
+// site of slice point r3 in the compact buffers and in the field
+__device__ __forceinline__ unsigned compact_of(const GaussArgs &g, const GaussSite &q, unsigned r3) {
+  return q.tt + g.tn * r3;
+}
+__device__ __forceinline__ unsigned field_of(const GaussArgs &g, const GaussSite &q, unsigned r3) {
+  return (g.tn > 1 ? q.tt : g.t0) + g.T * r3;
+}
+
+// One iteration: input from a compact buffer, output to a compact buffer
+// (OUT_FIELD false) or to the field (OUT_FIELD true).  U holds the links in
+// compact order.
+template <int P, bool OUT_FIELD>
+__global__ void __launch_bounds__(kBlock) k_gauss(GaussArgs g, SpinorIn<P> in, GaugeIn<P> U, SpinorOut<P> out,
+                                                  double alpha, double nrm) {
   using R = typename Prec<P>::R;
-  const unsigned j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (j >= nsites) return;
-  const unsigned s = t0 >= 0 ? (unsigned)t0 + a.L[0] * j : j;
-  const Site st = site_coords(s, a);
+  const GaussSite q = gauss_site(g);
+  if (!q.valid) return;
+  const SpinorIn<P> src = in.advanced(q.row);
+  const SpinorOut<P> dst = out.advanced(q.row);
+  const unsigned XY = g.X * g.Y;
+  const unsigned c = compact_of(g, q, q.r);
   Cx<R> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = Cx<R>{R(0), R(0)};
   Cx<R> psi[12], u[9], y[3];
-  auto step = [&](unsigned sf, unsigned sb, int nu) {
+  // rf, rb: forward and backward slice points in direction nu
+  auto step = [&](unsigned rf, unsigned rb, int nu) {
     Cx<R> up[12], ud[12];
-    in.load(sf, psi);
-    U.load(nu, s, u);
+    src.load(compact_of(g, q, rf), psi);
+    U.load(nu, c, u);
 #pragma unroll
     for (int sp = 0; sp < 4; sp++) {
       su3_mul<false>(y, u, &psi[3 * sp]);
 #pragma unroll
       for (int c = 0; c < 3; c++) up[3 * sp + c] = y[c];
     }
-    in.load(sb, psi);
-    U.load(nu, sb, u);
+    const unsigned cb = compact_of(g, q, rb);
+    src.load(cb, psi);
+    U.load(nu, cb, u);
 #pragma unroll
     for (int sp = 0; sp < 4; sp++) {
       su3_mul<true>(y, u, &psi[3 * sp]);
@@ -47,42 +110,141 @@ __global__ void __launch_bounds__(kBlock) k_gauss(DslashArgs a, SpinorIn<P> in, 
       acc[k].im += up[k].im + ud[k].im;
     }
   };
-  step(neighbour<1, +1>(st, a), neighbour<1, -1>(st, a), 1);
-  step(neighbour<2, +1>(st, a), neighbour<2, -1>(st, a), 2);
-  step(neighbour<3, +1>(st, a), neighbour<3, -1>(st, a), 3);
-  in.load(s, psi);
+  const unsigned r = q.r;
+  step(q.x == g.X - 1 ? r - (g.X - 1) : r + 1, q.x == 0 ? r + (g.X - 1) : r - 1, 1);
+  step(q.y == g.Y - 1 ? r - (g.Y - 1) * g.X : r + g.X, q.y == 0 ? r + (g.Y - 1) * g.X : r - g.X, 2);
+  step(q.z == g.Z - 1 ? r - (g.Z - 1) * XY : r + XY, q.z == 0 ? r + (g.Z - 1) * XY : r - XY, 3);
+  src.load(c, psi);
   const R al = (R)alpha, nr = (R)nrm;
 #pragma unroll
   for (int k = 0; k < 12; k++)
     psi[k] = Cx<R>{(psi[k].re + acc[k].re * al) * nr, (psi[k].im + acc[k].im * al) * nr};
-  out.store(s, psi);
+  dst.store(OUT_FIELD ? field_of(g, q, r) : c, psi);
 }
+
+// Stored values (and FP16 norms) of the slice's sites: field -> compact buffer.
+// src and dst point at right-hand side 0; rows lie 12 sstride / 12 dstride
+// elements apart.
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_gauss_gather(GaussArgs g, const typename Prec<P>::T *src,
+                                                         const float *snorm, size_t sstride,
+                                                         typename Prec<P>::T *dst, float *dnorm, size_t dstride) {
+  using T = typename Prec<P>::T;
+  constexpr int E = (int)sizeof(T);
+  const GaussSite q = gauss_site(g);
+  if (!q.valid) return;
+  const size_t s = field_of(g, q, q.r), c = compact_of(g, q, q.r);
+  const T *a = src + (size_t)q.row * 12 * sstride;
+  T *b = dst + (size_t)q.row * 12 * dstride;
+#pragma unroll
+  for (int k = 0; k < 12; k++) b[tile_index_e<E>(12, k, c)] = a[tile_index_e<E>(12, k, s)];
+  if (snorm) dnorm[(size_t)q.row * dstride + c] = snorm[(size_t)q.row * sstride + s];
+}
+
+// Stored values of the slice's links in directions 1..3: field -> compact
+// gauge field (one lane per slice site).
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_gauss_gather_links(GaussArgs g, const typename Prec<P>::T *src,
+                                                               size_t sstride, typename Prec<P>::T *dst,
+                                                               size_t dstride) {
+  using T = typename Prec<P>::T;
+  constexpr int E = (int)sizeof(T);
+  const unsigned r = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (r >= g.X * g.Y * g.Z) return;
+  const size_t s = g.t0 + (size_t)g.T * r;
+#pragma unroll
+  for (int k = 9; k < 36; k++) dst[gauge_index_e<E>(k, r, dstride)] = src[gauge_index_e<E>(k, s, sstride)];
+}
+
+// Rows of one pass: two compact buffers of `rows` right-hand sides each stay
+// within 4 GiB (at least one row); QCD_HIP_GAUSS_ROWS lowers the cap.
+static int gauss_rows(long row_bytes, int nrhs) {
+  const long cap = 4L << 30;
+  long rows = cap / (2 * row_bytes);
+  if (const char *e = std::getenv("QCD_HIP_GAUSS_ROWS")) {
+    const long r = std::atol(e);
+    if (r > 0 && r < rows) rows = r;
+  }
+  if (rows < 1) rows = 1;
+  return (int)(rows < nrhs ? rows : nrhs);
+}
+
+// lattice of the compact buffers and the compact links of one slice, kept
+// for the life of the process (guarded by state_mutex)
+static qcd_hip_lattice_s g_slice_lat;
+static qcd_hip_field g_slice_links = nullptr;
 
 template <int P>
-static int gauss_launch(const DslashArgs &a, qcd_hip_field in, int ri, qcd_hip_field g, qcd_hip_field out, int ro,
-                        double alpha, int t0, hipStream_t st) {
-  const unsigned n = t0 >= 0 ? a.V / a.L[0] : a.V;
-  k_gauss<P><<<grid_for(n), kBlock, 0, st>>>(a, SpinorIn<P>::of(in, ri), GaugeIn<P>::of(g), SpinorOut<P>::of(out, ro),
-                                             alpha, 1.0 / (1.0 + 6.0 * alpha), t0, n);
-  QH_CHECK(hipGetLastError());
+static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, int rows, double alpha, int t0,
+                     int niter, hipStream_t st) {
+  using T = typename Prec<P>::T;
+  const qcd_hip_lattice L = v->lat;
+  GaussArgs g;
+  g.T = L->L[0];
+  g.X = L->L[1];
+  g.Y = L->L[2];
+  g.Z = L->L[3];
+  g.tn = t0 >= 0 ? 1u : g.T;
+  g.t0 = t0 >= 0 ? (unsigned)t0 : 0u;
+  g.n = g.tn * g.X * g.Y * g.Z;
+  const double nrm = 1.0 / (1.0 + 6.0 * alpha);
+  const size_t vst = (size_t)L->stride, bst = (size_t)buf->lat->stride;
+  qcd_hip_field links = gauge;
+  if (t0 >= 0) {
+    if (g_slice_links && (g_slice_links->prec != v->prec || g_slice_links->lat->stride != buf->lat->stride)) {
+      qcd_hip_field_destroy(g_slice_links);
+      g_slice_links = nullptr;
+    }
+    if (!g_slice_links) QH_TRY(qcd_hip_gauge_create(&g_slice_links, buf->lat, v->prec));
+    g_slice_links->lat = buf->lat;
+    links = g_slice_links;
+    k_gauss_gather_links<P><<<grid_for((long)g.X * g.Y * g.Z), kBlock, 0, st>>>(
+        g, (const T *)gauge->data, vst, (T *)links->data, bst);
+    QH_CHECK(hipGetLastError());
+  }
+  const GaugeIn<P> U = GaugeIn<P>::of(links);
+  const unsigned nb = (unsigned)grid_for(g.n);
+  for (int r0 = 0; r0 < v->nrhs; r0 += rows) {
+    g.count = (unsigned)std::min(rows, v->nrhs - r0);
+    const unsigned grid = nb * g.count;
+    const SpinorOut<P> vout = SpinorOut<P>::of(v, r0);
+    const SpinorIn<P> ain = SpinorIn<P>::of(buf, 0), bin = SpinorIn<P>::of(buf, rows);
+    const SpinorOut<P> aout = SpinorOut<P>::of(buf, 0), bout = SpinorOut<P>::of(buf, rows);
+    k_gauss_gather<P><<<grid, kBlock, 0, st>>>(g, (const T *)v->data + (size_t)r0 * 12 * vst,
+                                               v->norm ? v->norm + (size_t)r0 * vst : nullptr, vst, (T *)buf->data,
+                                               buf->norm, bst);
+    QH_CHECK(hipGetLastError());
+    for (int it = 0; it < niter; it++) {
+      const bool from_a = (it % 2) == 0;
+      const SpinorIn<P> &src = from_a ? ain : bin;
+      if (it == niter - 1) k_gauss<P, true><<<grid, kBlock, 0, st>>>(g, src, U, vout, alpha, nrm);
+      else k_gauss<P, false><<<grid, kBlock, 0, st>>>(g, src, U, from_a ? bout : aout, alpha, nrm);
+      QH_CHECK(hipGetLastError());
+    }
+  }
   return 0;
 }
 
-int gauss_iterate(qcd_hip_field v, qcd_hip_field g, qcd_hip_field tmp, double alpha, int t0, int niter,
-                  hipStream_t st) {
-  DslashArgs a = make_args(v->lat, 0.0, 0.0);
-  for (int r = 0; r < v->nrhs; r++) {
-    for (int it = 0; it < niter; it++) {
-      const bool from_v = (it % 2) == 0;
-      qcd_hip_field in = from_v ? v : tmp, out = from_v ? tmp : v;
-      const int ri = from_v ? r : 0, ro = from_v ? 0 : r;
-      if (v->prec == QCD_HIP_F64) QH_TRY(gauss_launch<QCD_HIP_F64>(a, in, ri, g, out, ro, alpha, t0, st));
-      else if (v->prec == QCD_HIP_F32) QH_TRY(gauss_launch<QCD_HIP_F32>(a, in, ri, g, out, ro, alpha, t0, st));
-      else QH_TRY(gauss_launch<QCD_HIP_F16>(a, in, ri, g, out, ro, alpha, t0, st));
-    }
-    if (niter % 2) QH_TRY(lincomb_launch(v, r, 1.0, 0.0, tmp, 0, 0.0, 0.0, nullptr, 0, t0, st));
+int gauss_smear_field(qcd_hip_field v, qcd_hip_field gauge, double alpha, int t0, int niter, hipStream_t st) {
+  const qcd_hip_lattice L = v->lat;
+  qcd_hip_lattice bl = L;
+  if (t0 >= 0) {
+    qcd_hip_lattice_s &s = g_slice_lat;
+    s = qcd_hip_lattice_s();
+    s.L[0] = 1;
+    for (int d = 1; d < 4; d++) s.L[d] = L->L[d];
+    s.V = (long)L->L[1] * L->L[2] * L->L[3];
+    s.stride = round_up(s.V, 64);
+    for (int d = 0; d < 4; d++) s.ph_re[d] = 1.0;
+    bl = &g_slice_lat;
   }
-  return 0;
+  const long row_bytes = 12L * bl->stride * (long)spinor_comp_bytes(v->prec) + (v->prec == QCD_HIP_F16 ? 4L * bl->stride : 0);
+  const int rows = gauss_rows(row_bytes, v->nrhs);
+  qcd_hip_field buf = nullptr;
+  QH_TRY(scratch_spinor(0, bl, v->prec, 2 * rows, &buf));
+  if (v->prec == QCD_HIP_F64) return gauss_run<QCD_HIP_F64>(v, gauge, buf, rows, alpha, t0, niter, st);
+  if (v->prec == QCD_HIP_F32) return gauss_run<QCD_HIP_F32>(v, gauge, buf, rows, alpha, t0, niter, st);
+  return gauss_run<QCD_HIP_F16>(v, gauge, buf, rows, alpha, t0, niter, st);
 }
 
 }  // namespace qcdhip
@@ -107,9 +269,7 @@ int qcd_hip_gauss_smear(qcd_hip_field v, qcd_hip_field gauge, double alpha, int 
     return 1;
   }
   if (niter == 0) return 0;
-  qcd_hip_field tmp = nullptr;
-  QH_TRY(scratch_spinor(0, v->lat, v->prec, 1, &tmp));
-  QH_TRY(gauss_iterate(v, gauge, tmp, alpha, t < 0 ? -1 : t, niter, ctx()->stream));
+  QH_TRY(gauss_smear_field(v, gauge, alpha, t < 0 ? -1 : t, niter, ctx()->stream));
   QH_CHECK(hipStreamSynchronize(ctx()->stream));
   return 0;
 }

@@ -46,6 +46,34 @@ def test_gauss_point_source_50_iterations(gpu):
     assert rel_l2(f.download(), ref) <= 1e-13
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 2e-6), ("f16", 4e-3)])
+@pytest.mark.parametrize("rows", [None, "2"])
+def test_gauss_batched_rhs(gpu, monkeypatch, prec, tol, rows):
+    """Every right-hand side of a field smeared in one call, on a lattice whose
+    spatial extents all differ, one slice and all slices, odd and even iteration
+    counts; rows = 2 splits the five right-hand sides over three passes."""
+    Q = gpu
+    if rows:
+        monkeypatch.setenv("QCD_HIP_GAUSS_ROWS", rows)
+    Lb = (6, 4, 6, 8)
+    u = Q.gauge_generate(Lb, 0.25, 76)
+    geo = O.Geometry(Lb)
+    lat = Q.Lattice(Lb)
+    p = Q.PREC[prec]
+    g = Q.Gauge(lat, p).upload(u)
+    vs = [Q.spinor_gaussian(Lb, 760 + r) for r in range(5)]
+    for t, niter in [(4, 3), (1, 6), (-1, 2)]:
+        f = Q.Spinor(lat, p, nrhs=5)
+        for r, v in enumerate(vs):
+            f.upload(v, rhs=r)
+        Q.gauss_smear(f, g, 3.0, t, niter)
+        for r, v in enumerate(vs):
+            ref = v.copy()
+            for _ in range(niter):
+                ref = O.gauss3d(geo, ref, u, 3.0, t) if t >= 0 else O.gauss3d_all(geo, ref, u, 3.0)
+            assert rel_l2(f.download(rhs=r), ref) <= tol, (t, niter, r)
+
+
 def test_dropin_gauss(gpu):
     Q = gpu
     u = Q.gauge_generate(L, 0.25, 74)
