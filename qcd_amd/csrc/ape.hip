@@ -59,7 +59,7 @@ __device__ void project_su3(C m[9]) {
   for (int it = 0; it < 100; it++) {
     // y = (x^-1)^+ = conj(cofactor(x)) / conj(det x)
     const C d = det3(x);
-    const double dd = d.re * d.re + d.im * d.im;
+    const double rdd = 1.0 / (d.re * d.re + d.im * d.im);
     C y[9];
     double nx = 0.0, ny = 0.0;
 #pragma unroll
@@ -71,77 +71,26 @@ __device__ void project_su3(C m[9]) {
         const C s = cm(x[3 * r1 + c2], x[3 * r2 + c1]);
         const C cof{t.re - s.re, t.im - s.im};
         // cof / det, then conjugate
-        const C q{(cof.re * d.re + cof.im * d.im) / dd, (cof.im * d.re - cof.re * d.im) / dd};
+        const C q{(cof.re * d.re + cof.im * d.im) * rdd, (cof.im * d.re - cof.re * d.im) * rdd};
         y[3 * r + c] = cj(q);
         nx += x[3 * r + c].re * x[3 * r + c].re + x[3 * r + c].im * x[3 * r + c].im;
         ny += q.re * q.re + q.im * q.im;
       }
-    const double g = it < 8 ? sqrt(sqrt(ny / nx)) : 1.0;
+    const double g = it < 8 ? sqrt(sqrt(ny / nx)) : 1.0, rg = 1.0 / g;
     double diff = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-      const C n{0.5 * (g * x[k].re + y[k].re / g), 0.5 * (g * x[k].im + y[k].im / g)};
+      const C n{0.5 * (g * x[k].re + y[k].re * rg), 0.5 * (g * x[k].im + y[k].im * rg)};
       diff += (n.re - x[k].re) * (n.re - x[k].re) + (n.im - x[k].im) * (n.im - x[k].im);
       x[k] = n;
     }
-    if (it >= 8 && diff <= 1e-30) break;
+    if (diff <= 1e-30) break;   // a fixed point of the scaled step is unitary
   }
   const C d = det3(m);
   const double ph = atan2(d.im, d.re) / 3.0;
   const C e{cos(ph), -sin(ph)};
 #pragma unroll
   for (int k = 0; k < 9; k++) m[k] = cm(x[k], e);
-}
-
-__global__ void __launch_bounds__(kBlock) k_ape3d(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ out,
-                                                  size_t stride, double alpha) {
-  const unsigned j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (j >= 4 * a.V) return;
-  const unsigned s = j / 4, mu = j % 4;
-  const Site st = site_coords(s, a);
-  C m[9];
-  U.load(mu, s, m);
-  if (mu != 0) {
-    C S[9], A[9], B[9], T[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
-    // forward staples U_nu(x) U_mu(x+nu) U_nu(x+mu)^+, then backward ones
-    // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) (the reference's order)
-    for (int pass = 0; pass < 2; pass++)
-      for (unsigned nu = 1; nu < 4; nu++) {
-        if (nu == mu) continue;
-        unsigned sn, sm, sa;
-        if (pass == 0) {
-          sn = s;
-          sm = nu == 1 ? neighbour<1, +1>(st, a) : nu == 2 ? neighbour<2, +1>(st, a) : neighbour<3, +1>(st, a);
-          sa = mu == 1 ? neighbour<1, +1>(st, a) : mu == 2 ? neighbour<2, +1>(st, a) : neighbour<3, +1>(st, a);
-        } else {
-          sn = nu == 1 ? neighbour<1, -1>(st, a) : nu == 2 ? neighbour<2, -1>(st, a) : neighbour<3, -1>(st, a);
-          const Site sb = site_coords(sn, a);
-          sm = sn;
-          sa = mu == 1 ? neighbour<1, +1>(sb, a) : mu == 2 ? neighbour<2, +1>(sb, a) : neighbour<3, +1>(sb, a);
-        }
-        U.load(nu, sn, A);
-        U.load(mu, sm, B);
-        C W[9];
-        U.load(nu, sa, W);
-        if (pass == 0) {
-          mm3<false, false>(T, A, B);
-          mm3<false, true>(B, T, W);
-        } else {
-          mm3<false, false>(T, B, W);
-          mm3<true, false>(B, A, T);
-        }
-#pragma unroll
-        for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
-      }
-#pragma unroll
-    for (int k = 0; k < 9; k++) m[k] = C{m[k].re + alpha * S[k].re, m[k].im + alpha * S[k].im};
-    project_su3(m);
-  }
-  double2 *o = out + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
-#pragma unroll
-  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
 }
 
 // Plaquette sum of one site over the 6 planes mu < nu (qcd_observables.c:15-36):
@@ -249,6 +198,82 @@ __device__ __forceinline__ unsigned bwd_dir(unsigned mu, const Site &st, const D
                                            : mu == 2 ? neighbour<2, -1>(st, a) : neighbour<3, -1>(st, a);
 }
 
+// c += a b (DA: a^+, DB: b^+); each element's product is summed from zero
+// and then added, the rounding of computing a b first
+template <bool DA, bool DB>
+__device__ __forceinline__ void mm3_add(C c[9], const C a[9], const C b[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C s{0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const C x = DA ? cj(a[3 * k + i]) : a[3 * i + k];
+        const C y = DB ? cj(b[3 * j + k]) : b[3 * k + j];
+        const C p = cm(x, y);
+        s = C{s.re + p.re, s.im + p.im};
+      }
+      c[3 * i + j] = C{c[3 * i + j].re + s.re, c[3 * i + j].im + s.im};
+    }
+}
+
+// One APE step for the spatial directions, one lane per (site, direction)
+// with the direction mu = 1 + blockIdx.y uniform over the workgroup, so the
+// staple loops do not diverge.  Each staple's three links are loaded as its
+// products need them, which bounds the live matrices to the staple sum and
+// two factors.  The temporal links are copied by the caller.
+__global__ void __launch_bounds__(kBlock) k_ape3d(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ out,
+                                                  size_t stride, double alpha) {
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (s >= a.V) return;
+  const unsigned mu = 1 + blockIdx.y;
+  const Site st = site_coords(s, a);
+  C S[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
+  // forward staples U_nu(x) U_mu(x+nu) U_nu(x+mu)^+, then backward ones
+  // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) (the reference's order)
+  const unsigned sf = fwd_dir(mu, st, a);
+#pragma unroll 1
+  for (unsigned nu = 1; nu < 4; nu++) {
+    if (nu == mu) continue;
+    C T[9];
+    {
+      C A[9], B[9];
+      U.load(nu, s, A);
+      U.load(mu, fwd_dir(nu, st, a), B);
+      mm3<false, false>(T, A, B);
+    }
+    C W[9];
+    U.load(nu, sf, W);
+    mm3_add<false, true>(S, T, W);
+  }
+#pragma unroll 1
+  for (unsigned nu = 1; nu < 4; nu++) {
+    if (nu == mu) continue;
+    const unsigned sn = bwd_dir(nu, st, a);
+    C T[9];
+    {
+      C B[9], W[9];
+      U.load(mu, sn, B);
+      U.load(nu, fwd_dir(mu, site_coords(sn, a), a), W);
+      mm3<false, false>(T, B, W);
+    }
+    C A[9];
+    U.load(nu, sn, A);
+    mm3_add<true, false>(S, A, T);
+  }
+  C m[9];
+  U.load(mu, s, m);
+#pragma unroll
+  for (int k = 0; k < 9; k++) m[k] = C{m[k].re + alpha * S[k].re, m[k].im + alpha * S[k].im};
+  project_su3(m);
+  double2 *o = out + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
+#pragma unroll
+  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
+}
+
 // One stout step (qcd_smearing.c:374-476), one lane per (site, direction):
 // C = sum over nu != mu (nu from START) of the backward staple then the
 // forward one, W = C U^+, A = W - W^+ minus tr(A)/3, Q = -i rho/2 A,
@@ -325,8 +350,11 @@ int stout_step(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, hipStr
 
 int ape_step(qcd_hip_field out, qcd_hip_field in, double alpha, hipStream_t st) {
   const DslashArgs a = make_args(in->lat, 0.0, 0.0);
-  k_ape3d<<<grid_for(4 * in->lat->V), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
-                                                        (size_t)in->lat->stride, alpha);
+  // temporal links: direction 0 is the first 9 stride links of the layout
+  QH_CHECK(hipMemcpyAsync(out->data, in->data, (size_t)9 * in->lat->stride * sizeof(double2),
+                          hipMemcpyDeviceToDevice, st));
+  k_ape3d<<<dim3(grid_for(in->lat->V), 3), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                                           (size_t)in->lat->stride, alpha);
   QH_CHECK(hipGetLastError());
   return 0;
 }
