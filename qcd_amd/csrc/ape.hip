@@ -277,73 +277,84 @@ __global__ void __launch_bounds__(kBlock) k_ape3d(DslashArgs a, GaugeIn<QCD_HIP_
 // One stout step (qcd_smearing.c:374-476), one lane per (site, direction):
 // C = sum over nu != mu (nu from START) of the backward staple then the
 // forward one, W = C U^+, A = W - W^+ minus tr(A)/3, Q = -i rho/2 A,
-// U' = exp(iQ) U.  Directions below START (the temporal one for 3d stout)
-// are copied.
+// U' = exp(iQ) U.  The direction is START + blockIdx.y, uniform over the
+// workgroup; directions below START (the temporal one for 3d stout) are
+// copied by the caller.
 template <unsigned START>
 __global__ void __launch_bounds__(kBlock) k_stout(DslashArgs a, GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ out,
                                                   size_t stride, double rho) {
-  const unsigned j = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (j >= 4 * a.V) return;
-  const unsigned s = j / 4, mu = j % 4;
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (s >= a.V) return;
+  const unsigned mu = START + blockIdx.y;   // uniform over the workgroup
   const Site st = site_coords(s, a);
-  C m[9];
-  U.load(mu, s, m);
-  if (mu >= START) {
-    C S[9], A[9], B[9], T[9], W[9];
+  C S[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
-    const unsigned sf = fwd_dir(mu, st, a);
-    for (unsigned nu = START; nu < 4; nu++) {
-      if (nu == mu) continue;
-      // backward: U_nu(x-nu)^+ [U_mu(x-nu) U_nu(x-nu+mu)]
-      const unsigned sb = bwd_dir(nu, st, a);
-      const Site stb = site_coords(sb, a);
+  for (int k = 0; k < 9; k++) S[k] = C{0.0, 0.0};
+  const unsigned sf = fwd_dir(mu, st, a);
+#pragma unroll 1
+  for (unsigned nu = START; nu < 4; nu++) {
+    if (nu == mu) continue;
+    // backward: U_nu(x-nu)^+ [U_mu(x-nu) U_nu(x-nu+mu)]
+    const unsigned sb = bwd_dir(nu, st, a);
+    {
+      C T[9];
+      {
+        C B[9], W[9];
+        U.load(mu, sb, B);
+        U.load(nu, fwd_dir(mu, site_coords(sb, a), a), W);
+        mm3<false, false>(T, B, W);
+      }
+      C A[9];
       U.load(nu, sb, A);
-      U.load(mu, sb, B);
-      U.load(nu, fwd_dir(mu, stb, a), W);
-      mm3<false, false>(T, B, W);
-      mm3<true, false>(B, A, T);
-#pragma unroll
-      for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
-      // forward: [U_nu(x) U_mu(x+nu)] U_nu(x+mu)^+
-      U.load(nu, s, A);
-      U.load(mu, fwd_dir(nu, st, a), B);
-      U.load(nu, sf, W);
-      mm3<false, false>(T, A, B);
-      mm3<false, true>(B, T, W);
-#pragma unroll
-      for (int k = 0; k < 9; k++) S[k] = C{S[k].re + B[k].re, S[k].im + B[k].im};
+      mm3_add<true, false>(S, A, T);
     }
-    mm3<false, true>(W, S, m);   // W = C U^+
-    C q[9];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) q[3 * r + c] = C{W[3 * r + c].re - W[3 * c + r].re, W[3 * r + c].im + W[3 * c + r].im};
-    const C tr{(q[0].re + q[4].re + q[8].re) / 3.0, (q[0].im + q[4].im + q[8].im) / 3.0};
-#pragma unroll
-    for (int r = 0; r < 3; r++) q[4 * r] = C{q[4 * r].re - tr.re, q[4 * r].im - tr.im};
-#pragma unroll
-    for (int k = 0; k < 9; k++) q[k] = C{0.5 * rho * q[k].im, -0.5 * rho * q[k].re};   // Q = -i rho/2 A
-    stout_exp(q);
-    mm3<false, false>(T, q, m);
-#pragma unroll
-    for (int k = 0; k < 9; k++) m[k] = T[k];
+    // forward: [U_nu(x) U_mu(x+nu)] U_nu(x+mu)^+
+    {
+      C T[9];
+      {
+        C A[9], B[9];
+        U.load(nu, s, A);
+        U.load(mu, fwd_dir(nu, st, a), B);
+        mm3<false, false>(T, A, B);
+      }
+      C W[9];
+      U.load(nu, sf, W);
+      mm3_add<false, true>(S, T, W);
+    }
   }
+  C m[9], W[9];
+  U.load(mu, s, m);
+  mm3<false, true>(W, S, m);   // W = C U^+
+  C q[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) q[3 * r + c] = C{W[3 * r + c].re - W[3 * c + r].re, W[3 * r + c].im + W[3 * c + r].im};
+  const C tr{(q[0].re + q[4].re + q[8].re) / 3.0, (q[0].im + q[4].im + q[8].im) / 3.0};
+#pragma unroll
+  for (int r = 0; r < 3; r++) q[4 * r] = C{q[4 * r].re - tr.re, q[4 * r].im - tr.im};
+#pragma unroll
+  for (int k = 0; k < 9; k++) q[k] = C{0.5 * rho * q[k].im, -0.5 * rho * q[k].re};   // Q = -i rho/2 A
+  stout_exp(q);
+  mm3<false, false>(W, q, m);
   double2 *o = out + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
 #pragma unroll
-  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
+  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(W[k].re, W[k].im);
 }
 
 int stout_step(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, hipStream_t st) {
   const DslashArgs a = make_args(in->lat, 0.0, 0.0);
-  const int grid = grid_for(4 * in->lat->V);
-  if (ndim == 3)
-    k_stout<1><<<grid, kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
-                                        (size_t)in->lat->stride, rho);
-  else
-    k_stout<0><<<grid, kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
-                                        (size_t)in->lat->stride, rho);
+  const int grid = grid_for(in->lat->V);
+  if (ndim == 3) {
+    // temporal links copied: direction 0 is the first 9 stride links of the layout
+    QH_CHECK(hipMemcpyAsync(out->data, in->data, (size_t)9 * in->lat->stride * sizeof(double2),
+                            hipMemcpyDeviceToDevice, st));
+    k_stout<1><<<dim3(grid, 3), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                                 (size_t)in->lat->stride, rho);
+  } else {
+    k_stout<0><<<dim3(grid, 4), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F64>::of(in), (double2 *)out->data,
+                                                 (size_t)in->lat->stride, rho);
+  }
   QH_CHECK(hipGetLastError());
   return 0;
 }
