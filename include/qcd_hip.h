@@ -108,7 +108,11 @@ int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, in
 /* ------------------------------------------------------------ smearing
  * niter iterations of 3D Gaussian smearing of every rhs of v on time-slice t
  * (t < 0: every time-slice), in place (lib/src/qcd_smearing.c:25-199).
- * v and gauge must have the same precision. */
+ * v and gauge must have the same precision.  The iterations run on two
+ * library-owned buffers of the slice (the volume when t < 0) per rhs, as many
+ * rhs per pass as fit in 4 GiB and at least one (QCD_HIP_GAUSS_ROWS lowers
+ * the count), plus one copy of the slice's links; both stay allocated for
+ * reuse until the process ends. */
 int qcd_hip_gauss_smear(qcd_hip_field v, qcd_hip_field gauge, double alpha, int t, int niter);
 
 /* ------------------------------------------------------------ BLAS (FP64
