@@ -97,7 +97,8 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
  * steps along z carrying the backward z hop in registers, 3 | (n << 8) = the
  * same over tiles of whole t-rows staged in LDS with their x halo rows,
  * 4 | (n << 8) = traversal 2 with the forward z neighbour kept in LDS as the
- * next step's centre) and occupancy variant (0 = compiler default, 2 or 3 =
+ * next step's centre, 5 | (n << 8) = traversal 2 with two rhs per lane for
+ * FP32/FP16 batches with the D or Q epilogue) and occupancy variant (0 = compiler default, 2 or 3 =
  * at least that many waves per SIMD, 16 = scheduling fences between hop
  * directions) on these fields and keep the fastest for their (lattice
  * extents, precisions).  Runs by itself on first use unless the environment
