@@ -235,24 +235,44 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
 
-// full FP64 vector f0 + ids[row] -> half vectors ev[slot], od[slot]
-// One lane per full-lattice site s (consecutive lanes, consecutive sites, so
-// the full vector is read and written in whole segments); its half-lattice
-// entry is s >> 1 of the field of its parity.
+// Parity of full-lattice site s; its half-lattice entry is s >> 1 of the
+// field of that parity.
 __device__ __forceinline__ unsigned site_parity(unsigned s, const DslashArgs &a) {
   const Site st = site_coords(s, a);
   return (st.t + st.x + st.y + st.z) & 1u;
 }
 
+// One lane per FP64 element of the full vector: lane j holds component
+// (j % 768) / 64 of site (j / 768) * 64 + j % 64, so every lane issues one
+// 16-byte load or store per vector and a wavefront covers one contiguous
+// 1 KB plane of a tile (a copy of this form streams at 6.1 TB/s on MI355X,
+// against 5.35 TB/s for 12 components per lane).
+struct EoElem {
+  unsigned s, k;
+  bool valid;
+};
+__device__ __forceinline__ EoElem eo_elem(const EoArgs &e) {
+  const unsigned j = blockIdx.x * kBlock + threadIdx.x;
+  EoElem q;
+  q.s = (j / (12u * kTile)) * kTile + j % kTile;
+  q.k = (j % (12u * kTile)) / kTile;
+  q.valid = q.s < e.d.V;
+  return q;
+}
+// element index of component k of site s in an FP64 vector
+__device__ __forceinline__ size_t eo_at(unsigned k, unsigned s) {
+  return (size_t)(s / kTile) * 12 * kTile + (size_t)k * kTile + s % kTile;
+}
+
+// full FP64 vector f0 + ids[row] -> half vectors ev[slot], od[slot]
 __global__ void __launch_bounds__(kBlock) k_eo_split(EoArgs e, SpinorSet<QCD_HIP_F64> full, int f0,
                                                      SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
                                                      const int *ids) {
   const int b = ids[blockIdx.y];
-  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= e.d.V) return;
-  Cx<double> v[12];
-  full.load(f0 + b, s, v);
-  (site_parity(s, e.d) ? od : ev).store(b, s >> 1, v);
+  const EoElem q = eo_elem(e);
+  if (!q.valid) return;
+  const SpinorSet<QCD_HIP_F64> &h = site_parity(q.s, e.d) ? od : ev;
+  h.base[(size_t)b * h.vstride + eo_at(q.k, q.s >> 1)] = full.base[(size_t)(f0 + b) * full.vstride + eo_at(q.k, q.s)];
 }
 
 // x[f0 + slot] += (ev[slot] on even sites, od[slot] on odd sites)
@@ -260,14 +280,12 @@ __global__ void __launch_bounds__(kBlock) k_eo_accumulate(EoArgs e, SpinorSet<QC
                                                           SpinorSet<QCD_HIP_F64> ev, SpinorSet<QCD_HIP_F64> od,
                                                           const int *ids) {
   const int b = ids[blockIdx.y];
-  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= e.d.V) return;
-  Cx<double> xv[12], d[12];
-  x.load(f0 + b, s, xv);
-  (site_parity(s, e.d) ? od : ev).load(b, s >> 1, d);
-#pragma unroll
-  for (int k = 0; k < 12; k++) xv[k] = Cx<double>{xv[k].re + d[k].re, xv[k].im + d[k].im};
-  x.store(f0 + b, s, xv);
+  const EoElem q = eo_elem(e);
+  if (!q.valid) return;
+  const SpinorSet<QCD_HIP_F64> &h = site_parity(q.s, e.d) ? od : ev;
+  double2 &xv = x.base[(size_t)(f0 + b) * x.vstride + eo_at(q.k, q.s)];
+  const double2 d = h.base[(size_t)b * h.vstride + eo_at(q.k, q.s >> 1)];
+  xv = make_double2(xv.x + d.x, xv.y + d.y);
 }
 
 // out[slot] = M^-1 in[slot] (FP64 half vectors)
@@ -411,7 +429,7 @@ int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_
 int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
              hipStream_t st) {
   const EoArgs e = make_eo_args(full->lat, 0.0, 0.0, 0);
-  k_eo_split<<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(full), f0,
+  k_eo_split<<<dim3(grid_for(12 * round_up(e.d.V, kTile)), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(full), f0,
                                                               SpinorSet<QCD_HIP_F64>::of(ev),
                                                               SpinorSet<QCD_HIP_F64>::of(od), ids);
   QH_CHECK(hipGetLastError());
@@ -421,7 +439,7 @@ int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, con
 int eo_accumulate(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
                   hipStream_t st) {
   const EoArgs e = make_eo_args(x->lat, 0.0, 0.0, 0);
-  k_eo_accumulate<<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(x), f0,
+  k_eo_accumulate<<<dim3(grid_for(12 * round_up(e.d.V, kTile)), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(x), f0,
                                                                    SpinorSet<QCD_HIP_F64>::of(ev),
                                                                    SpinorSet<QCD_HIP_F64>::of(od), ids);
   QH_CHECK(hipGetLastError());
