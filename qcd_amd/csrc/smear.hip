@@ -173,6 +173,7 @@ static int gauss_rows(long row_bytes, int nrhs) {
 // for the life of the process (guarded by state_mutex)
 static qcd_hip_lattice_s g_slice_lat;
 static qcd_hip_field g_slice_links = nullptr;
+static long g_slice_links_stride = 0;   // stride g_slice_links was allocated for
 
 template <int P>
 static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, int rows, double alpha, int t0,
@@ -191,11 +192,16 @@ static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, in
   const size_t vst = (size_t)L->stride, bst = (size_t)buf->lat->stride;
   qcd_hip_field links = gauge;
   if (t0 >= 0) {
-    if (g_slice_links && (g_slice_links->prec != v->prec || g_slice_links->lat->stride != buf->lat->stride)) {
+    // g_slice_lat is rewritten on every call, so the allocation's own stride
+    // is kept apart from it
+    if (g_slice_links && (g_slice_links->prec != v->prec || g_slice_links_stride != buf->lat->stride)) {
       qcd_hip_field_destroy(g_slice_links);
       g_slice_links = nullptr;
     }
-    if (!g_slice_links) QH_TRY(qcd_hip_gauge_create(&g_slice_links, buf->lat, v->prec));
+    if (!g_slice_links) {
+      QH_TRY(qcd_hip_gauge_create(&g_slice_links, buf->lat, v->prec));
+      g_slice_links_stride = buf->lat->stride;
+    }
     g_slice_links->lat = buf->lat;
     links = g_slice_links;
     k_gauss_gather_links<P><<<grid_for((long)g.X * g.Y * g.Z), kBlock, 0, st>>>(

@@ -74,6 +74,22 @@ def test_gauss_batched_rhs(gpu, monkeypatch, prec, tol, rows):
             assert rel_l2(f.download(rhs=r), ref) <= tol, (t, niter, r)
 
 
+def test_gauss_slices_of_growing_size(gpu):
+    """The library keeps the slice buffers and the slice's links between
+    calls; a later call on a larger slice must get larger buffers."""
+    Q = gpu
+    for k, Ls in enumerate([(4, 4, 2, 2), (4, 8, 8, 6), (6, 4, 4, 4), (4, 10, 8, 8)]):
+        u = Q.gauge_generate(Ls, 0.25, 780 + k)
+        v = Q.spinor_gaussian(Ls, 790 + k)
+        lat = Q.Lattice(Ls)
+        g = Q.Gauge(lat).upload(u)
+        f = Q.Spinor(lat).upload(v)
+        Q.gauss_smear(f, g, 2.0, 1, 2)
+        geo = O.Geometry(Ls)
+        ref = O.gauss3d(geo, O.gauss3d(geo, v, u, 2.0, 1), u, 2.0, 1)
+        assert rel_l2(f.download(), ref) <= 1e-13, Ls
+
+
 def test_dropin_gauss(gpu):
     Q = gpu
     u = Q.gauge_generate(L, 0.25, 74)
