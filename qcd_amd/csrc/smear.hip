@@ -188,6 +188,7 @@ static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, in
   g.tn = t0 >= 0 ? 1u : g.T;
   g.t0 = t0 >= 0 ? (unsigned)t0 : 0u;
   g.n = g.tn * g.X * g.Y * g.Z;
+  g.count = 0;   // set per pass below
   const double nrm = 1.0 / (1.0 + 6.0 * alpha);
   const size_t vst = (size_t)L->stride, bst = (size_t)buf->lat->stride;
   qcd_hip_field links = gauge;
