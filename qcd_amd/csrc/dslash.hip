@@ -91,7 +91,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
 #pragma unroll
       for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     }
-    out.store(s, o);
+    out.template store<true>(s, o);
   }
   if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -157,7 +157,9 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
           in.load_raw(s, rc, nc);
         }
         local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
-        U.load_raw(3, s, u);
+        // U_z(s) is read by this lane only (the next step carries the hop it
+        // feeds): a streaming load leaves L2 to the spinor planes
+        U.template load_raw<true>(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
         hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
@@ -188,7 +190,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 #pragma unroll
         for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
       }
-      out.store(s, o);
+      out.template store<true>(s, o);
     }
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
@@ -237,7 +239,7 @@ __device__ __forceinline__ void finish_site(const Cx<R> acc[4][3], const SpinorO
 #pragma unroll
     for (int c = 0; c < 3; c++) o[3 * sp + c] = Cx<R>{R(-0.5) * acc[sp][c].re, R(-0.5) * acc[sp][c].im};
   apply_epilogue<EPI>(o, b, s);
-  out.store(s, o);
+  out.template store<true>(s, o);
 }
 
 // MAP_ZMARCH2: the z-march of k_dslash_zm with two right-hand sides per lane
@@ -415,7 +417,9 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
       {
         typename Prec<PG>::T u[9];
         local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
-        U.load_raw(3, s, u);
+        // U_z(s) is read by this lane only (the next step carries the hop it
+        // feeds): a streaming load leaves L2 to the spinor planes
+        U.template load_raw<true>(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
         hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
@@ -460,7 +464,7 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 #pragma unroll
         for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
       }
-      out.store(s, o);
+      out.template store<true>(s, o);
     }
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);

@@ -66,6 +66,26 @@ template <typename T>
 __device__ __forceinline__ T ld_plane(const T *base, unsigned off) {
   return *(const T *)((const char *)base + off);
 }
+// the same load with the nontemporal (streaming) cache hint
+template <typename T>
+__device__ __forceinline__ T ld_plane_nt(const T *base, unsigned off) {
+  const T *p = (const T *)((const char *)base + off);
+  if constexpr (sizeof(T) == 16) {
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const v2d v = __builtin_nontemporal_load((const v2d *)p);
+    T r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+  } else {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    const v2f v = __builtin_nontemporal_load((const v2f *)p);
+    T r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+  }
+}
 template <typename T>
 __device__ __forceinline__ void st_plane(T *base, unsigned off, T v) {
   *(T *)((char *)base + off) = v;
@@ -192,6 +212,9 @@ struct SpinorOut {
     return s;
   }
   __device__ __forceinline__ T *comp(int k) const { return p[k / kPer] + (k % kPer) * kTile; }
+  // NT: nontemporal (streaming) stores, for outputs no later kernel of the
+  // same pass reads back through L2
+  template <bool NT = false>
   __device__ __forceinline__ void store(unsigned s, const Cx<R> v[12]) const {
     const unsigned off = SpinorIn<P>::offset(s);
     if constexpr (P == QCD_HIP_F16) {
@@ -202,23 +225,36 @@ struct SpinorOut {
       // stored as zero: 1/mx would overflow and 0 * inf give NaN
       if (!(mx >= FLT_MIN)) mx = R(0);
       const R inv = mx > R(0) ? R(1) / mx : R(0);
-      st_plane(norm, s * 4u, (float)mx);
+      if constexpr (NT) __builtin_nontemporal_store((float)mx, (float *)((char *)norm + s * 4u));
+      else st_plane(norm, s * 4u, (float)mx);
 #pragma unroll
       for (int g = 0; g < 3; g++) {
         u32x4 q;
 #pragma unroll
         for (int i = 0; i < 4; i++)
           q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
-        *(u32x4 *)((char *)p[0] + g * 4 * kTile * 4 + off) = q;
+        u32x4 *d = (u32x4 *)((char *)p[0] + g * 4 * kTile * 4 + off);
+        if constexpr (NT) __builtin_nontemporal_store(q, d);
+        else *d = q;
       }
     } else if constexpr (P == QCD_HIP_F32) {
 #pragma unroll
-      for (int g = 0; g < 6; g++)
-        *(float4 *)((char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off) =
-            make_float4((float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im);
+      for (int g = 0; g < 6; g++) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const v4f q{(float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im};
+        v4f *d = (v4f *)((char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off);
+        if constexpr (NT) __builtin_nontemporal_store(q, d);
+        else *d = q;
+      }
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(comp(k), off, from_cx<T>(v[k]));
+      for (int k = 0; k < 12; k++) {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        const v2d q{(double)v[k].re, (double)v[k].im};
+        v2d *d = (v2d *)((char *)comp(k) + off);
+        if constexpr (NT) __builtin_nontemporal_store(q, d);
+        else *d = q;
+      }
     }
   }
 };
@@ -236,7 +272,9 @@ struct GaugeIn {
     return g;
   }
   __device__ __forceinline__ const T *comp(int mu, int k) const { return p[kNb * mu + k / kPer] + (k % kPer) * kTile; }
-  // stored values of the link of direction mu at site s
+  // stored values of the link of direction mu at site s (NT: streaming
+  // cache hint, FP32/FP64)
+  template <bool NT = false>
   __device__ __forceinline__ void load_raw(int mu, unsigned s, T r[9]) const {
     if constexpr (P == QCD_HIP_F16) {
       const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile) * 3u) * 4u;
@@ -250,7 +288,7 @@ struct GaugeIn {
     } else {
       const unsigned off = ((s / kTile) * (9u * kTile) + (s % kTile)) * (unsigned)sizeof(T);
 #pragma unroll
-      for (int k = 0; k < 9; k++) r[k] = ld_plane(comp(mu, k), off);
+      for (int k = 0; k < 9; k++) r[k] = NT ? ld_plane_nt(comp(mu, k), off) : ld_plane(comp(mu, k), off);
     }
   }
   template <typename R>
