@@ -187,10 +187,10 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop2(EoArgs e, SpinorIn<PS> in_al
   eo_hop_dir2<3, PS, PG, R>(acc0, acc1, a, st, v0.in, v1.in, U);
   Cx<R> o[12];
   eo_epilogue<EPI>(o, acc0, e, v0.b, h);
-  v0.out.store(h, o);
+  v0.out.template store<true>(h, o);
   if (r1 != r0) {
     eo_epilogue<EPI>(o, acc1, e, v1.b, h);
-    v1.out.store(h, o);
+    v1.out.template store<true>(h, o);
   }
 }
 
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all
 #pragma unroll
       for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     }
-    rv.out.store(h, o);
+    rv.out.template store<true>(h, o);
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
