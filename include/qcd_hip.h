@@ -105,6 +105,9 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
  * sets QCD_HIP_AUTOTUNE=0.  All candidates give bit-identical results. */
 int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, int tuned[2], double *ms);
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ);
+/* The traversal candidates (map values above) the autotuner times: fills up
+ * to max entries of maps and returns how many there are. */
+int qcd_hip_dslash_candidates(int *maps, int max);
 
 /* ------------------------------------------------------------ smearing
  * niter iterations of 3D Gaussian smearing of every rhs of v on time-slice t
@@ -164,6 +167,10 @@ typedef struct {
 
 void qcd_hip_solver_default_params(qcd_hip_solver_params *p);
 int  qcd_hip_solver_create(qcd_hip_solver *s, qcd_hip_lattice lat, const qcd_hip_solver_params *p);
+/* Device bytes qcd_hip_solver_create plus qcd_hip_solver_set_gauge allocate
+ * for a lattice of extents L with parameters p (0 for invalid arguments);
+ * the serving loop admits requests by this figure. */
+size_t qcd_hip_solver_bytes(const int L[4], const qcd_hip_solver_params *p);
 void qcd_hip_solver_destroy(qcd_hip_solver s);
 /* set (or refresh after re-upload) the FP64 gauge field; builds the inner-
  * precision copy */

@@ -184,8 +184,11 @@ int qcd_applyWilsonTMOpNoOptim(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u,
 int qcd_applyQTMOp(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_real_8 m, qcd_real_8 tm);
 
 /* Extension without a reference counterpart: keep u resident in HBM so that
- * the drop-in operators, smearing and BLAS above skip its upload.  Call again
- * after changing u->D; unbind to release the device copy. */
+ * the drop-in operators, smearing and BLAS above skip its upload.  Library
+ * calls that write u (qcd_apeSmear3d, qcd_stoutsmearing, qcd_projectSU33d,
+ * qcd_copyGaugeField, qcd_zeroGaugeField, qcd_getGaugeLime) refresh the
+ * device copy themselves; call again after changing u->D by other means;
+ * unbind to release the device copy. */
 int  qcd_hip_bindGaugeField(qcd_gaugeField *u);
 void qcd_hip_unbindGaugeField(void);
 

@@ -128,6 +128,7 @@ def lib():
         "qcd_hip_dslash_time": (i, [_vp, _vp, _vp, d, d, i, i, i, _dp]),
         "qcd_hip_dslash_tune": (i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _dp]),
         "qcd_hip_dslash_set_config": (i, [_vp, _vp, i, i]),
+        "qcd_hip_dslash_candidates": (i, [ctypes.POINTER(ctypes.c_int), i]),
         "qcd_hip_gauss_smear": (i, [_vp, _vp, d, i, i]),
         "qcd_hip_norm2": (i, [_vp, i, _dp]),
         "qcd_hip_dot": (i, [_vp, i, _vp, i, _dp]),
@@ -136,6 +137,7 @@ def lib():
         "qcd_hip_solver_default_params": (None, [ctypes.POINTER(SolverParams)]),
         "qcd_hip_solver_create": (i, [ctypes.POINTER(_vp), _vp, ctypes.POINTER(SolverParams)]),
         "qcd_hip_solver_destroy": (None, [_vp]),
+        "qcd_hip_solver_bytes": (sz, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(SolverParams)]),
         "qcd_hip_solver_set_gauge": (i, [_vp, _vp]),
         "qcd_hip_solver_solve": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverStats)]),
         "qcd_hip_invert": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverParams), ctypes.POINTER(SolverStats)]),
@@ -171,6 +173,8 @@ def lib():
         "qcd_projectSU33d": (None, [pf]),
         "qcd_gaussIteration3dAll": (i, [pf, pf, d, ctypes.c_uint16]),
         "qcd_zeroVector": (None, [pf]),
+        "qcd_zeroGaugeField": (None, [pf]),
+        "qcd_copyGaugeField": (None, [pf, pf]),
         "qcd_copyVector": (None, [pf, pf]),
         "qcd_copyVectorPropagator": (None, [pf, pf, ctypes.c_uint16, ctypes.c_uint16]),
         "qcd_copyPropagatorVector": (None, [pf, pf, ctypes.c_uint16, ctypes.c_uint16]),
@@ -406,6 +410,14 @@ def dslash_tune(out, inp, gauge):
     return int(cfg[0]), int(cfg[1]), float(ms[0])
 
 
+def dslash_candidates():
+    """The traversal candidates (map values) the autotuner times."""
+    n = lib().qcd_hip_dslash_candidates(None, 0)
+    buf = (ctypes.c_int * n)()
+    lib().qcd_hip_dslash_candidates(buf, n)
+    return list(buf)
+
+
 def dslash_set_config(inp, gauge, map_, occ):
     _check(lib().qcd_hip_dslash_set_config(inp.h, gauge.h, int(map_), int(occ)), "qcd_hip_dslash_set_config")
 
@@ -442,6 +454,12 @@ def solver_params(**kw):
             v = OP_Q if v == "Q" else OP_D
         setattr(p, k, v)
     return p
+
+
+def solver_bytes(L, **kw):
+    """Device bytes a Solver with these parameters allocates on lattice extents L."""
+    p = solver_params(**kw)
+    return int(lib().qcd_hip_solver_bytes((ctypes.c_int * 4)(*[int(x) for x in L]), ctypes.byref(p)))
 
 
 class Solver:

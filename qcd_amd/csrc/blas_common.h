@@ -98,6 +98,11 @@ int gauss_smear_field(qcd_hip_field v, qcd_hip_field g, double alpha, int t0, in
 // even-odd preconditioning applies (every extent even), eo.hip
 bool eo_supported(qcd_hip_lattice lat);
 
+// device bytes a solver with these parameters allocates on a lattice of
+// extents L (qcd_hip_solver_create plus the inner-precision links of
+// qcd_hip_solver_set_gauge), solver.hip
+size_t solver_device_bytes(const int L[4], const qcd_hip_solver_params *prm);
+
 // device-side synthetic inputs (asynchronous on st)
 int gauge_generate_device(qcd_hip_field g, double eps, uint64_t seed, hipStream_t st);
 int point_source_device(qcd_hip_field f, int rhs, long site, int spin, int col, hipStream_t st);

@@ -72,6 +72,20 @@ int gauge_for(qcd_gaugeField *u, qcd_hip_lattice lat, qcd_hip_field *out) {
   return 0;
 }
 
+}  // namespace
+
+namespace qcdhip {
+// A drop-in call wrote new links into the host field whose data start at
+// `host`: when that field is the bound (resident) one, its device copy is
+// refreshed, so later calls on the field see the new links.
+int dropin_gauge_written(const void *host) {
+  if (!g_dc.bound || !host || g_dc.bound_host != host) return 0;
+  return qcd_hip_gauge_upload(g_dc.bound, (const double *)host);
+}
+}  // namespace qcdhip
+
+namespace {
+
 // scratch spinors 4..7 are reserved for the drop-in layer
 int spinor_in(int slot, qcd_hip_lattice lat, const qcd_vector *v, qcd_hip_field *out) {
   QH_TRY(scratch_spinor(4 + slot, lat, QCD_HIP_F64, 1, out));
@@ -117,7 +131,9 @@ int apply_operator(const char *who, int op, qcd_vector *Dv, qcd_vector *v, qcd_g
   if (check_operator_args(who, Dv, v, u)) return 1;
   qcd_hip_lattice lat;
   qcd_hip_field g, in, out;
-  int rc = lattice_for(v->geo, &lat);
+  // boundary phases come from the gauge field's geometry
+  // (lib/src/qcd_wilson.c:174-177 reads u->geo->theta)
+  int rc = lattice_for(u->geo, &lat);
   if (!rc) rc = gauge_for(u, lat, &g);
   if (!rc) rc = spinor_in(0, lat, v, &in);
   if (!rc) rc = spinor_in(1, lat, nullptr, &out);
@@ -381,6 +397,7 @@ int qcd_apeSmear3d(qcd_gaugeField *apeu, qcd_gaugeField *u, qcd_real_8 alpha) {
   if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
   if (!rc) rc = qcd_hip_ape_smear(out, g, alpha, 1);
   if (!rc) rc = qcd_hip_gauge_download(out, (double *)apeu->D);
+  if (!rc) rc = dropin_gauge_written(apeu->D);
   qcd_hip_field_destroy(out);
   return report("qcd_apeSmear3d", rc);
 }
@@ -402,6 +419,7 @@ int qcd_stoutsmearing(qcd_gaugeField *u_out, qcd_gaugeField *u, qcd_real_8 rho, 
   if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
   if (!rc) rc = qcd_hip_stout_smear(out, g, rho, ndim, 1);
   if (!rc) rc = qcd_hip_gauge_download(out, (double *)u_out->D);
+  if (!rc) rc = dropin_gauge_written(u_out->D);
   qcd_hip_field_destroy(out);
   return report("qcd_stoutsmearing", rc);
 }
@@ -417,6 +435,7 @@ void qcd_projectSU33d(qcd_gaugeField *gf) {
   if (!rc) rc = qcd_hip_gauge_create(&out, lat, QCD_HIP_F64);
   if (!rc) rc = qcd_hip_ape_smear(out, g, 0.0, 1);
   if (!rc) rc = qcd_hip_gauge_download(out, (double *)gf->D);
+  if (!rc) rc = dropin_gauge_written(gf->D);
   qcd_hip_field_destroy(out);
   report("qcd_projectSU33d", rc);
 }
@@ -441,7 +460,10 @@ qcd_real_8 qcd_calculatePlaquette(qcd_gaugeField *u) {
 // host data movement: lib/src/qcd_blas.c:137-233
 void qcd_zeroVector(qcd_vector *vec) { std::memset(vec->D, 0, 12 * vec->geo->lV * sizeof(qcd_complex_16)); }
 void qcd_zeroPropagator(qcd_propagator *p) { std::memset(p->D, 0, 144 * p->geo->lV * sizeof(qcd_complex_16)); }
-void qcd_zeroGaugeField(qcd_gaugeField *u) { std::memset(u->D, 0, 36 * u->geo->lV * sizeof(qcd_complex_16)); }
+void qcd_zeroGaugeField(qcd_gaugeField *u) {
+  std::memset(u->D, 0, 36 * u->geo->lV * sizeof(qcd_complex_16));
+  report("qcd_zeroGaugeField", dropin_gauge_written(u->D));
+}
 
 void qcd_setVector(qcd_vector *vec, qcd_complex_16 c) {
   qcd_complex_16 *p = &vec->D[0][0][0];
@@ -454,6 +476,7 @@ void qcd_copyVector(qcd_vector *dest, qcd_vector *src) {
 
 void qcd_copyGaugeField(qcd_gaugeField *dest, qcd_gaugeField *src) {
   std::memcpy(dest->D, src->D, 36 * dest->geo->lV * sizeof(qcd_complex_16));
+  report("qcd_copyGaugeField", dropin_gauge_written(dest->D));
 }
 
 void qcd_copyVectorPropagator(qcd_vector *vec, qcd_propagator *prop, qcd_uint_2 nu, qcd_uint_2 c2) {

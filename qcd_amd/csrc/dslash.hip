@@ -938,6 +938,14 @@ int qcd_hip_dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
   return 0;
 }
 
+// The traversal candidates the autotuner times (before the two-rhs batch
+// candidates): fills up to max entries of maps, returns how many there are.
+int qcd_hip_dslash_candidates(int *maps, int max) {
+  const int n = (int)(sizeof(kMaps) / sizeof(kMaps[0]));
+  for (int i = 0; maps && i < n && i < max; i++) maps[i] = kMaps[i];
+  return n;
+}
+
 // Force a configuration for these fields' key: map = 0 contiguous per XCD,
 // 1 y-slab per XCD, 2 | (n << 8) z-march of n steps, 3 | (n << 8) z-march of
 // n steps with LDS-staged t and x neighbours, 4 | (n << 8) z-march with the

@@ -33,6 +33,12 @@ extern "C" {
 #include "qcd_io.h"
 }
 
+namespace qcdhip {
+// dropin.cpp: refreshes the resident copy of a bound gauge field after its
+// host data changed
+int dropin_gauge_written(const void *host);
+}
+
 namespace {
 
 constexpr uint32_t kLimeMagic = 1164413355u;
@@ -300,6 +306,10 @@ int qcd_getGaugeLime(char *fname, qcd_gaugeField *u) {
             double *d = &u->D[lexic(t, x, y, z, L)][(mu + 1) % 4][0][0].re;
             for (int k = 0; k < 18; k++, p += w) d[k] = w == 8 ? be_f64(p) : be_f32(p);
           }
+  if (qcdhip::dropin_gauge_written(u->D)) {
+    std::fprintf(stderr, "process 0: Error in qcd_getGaugeLime! Could not refresh the bound device copy\n");
+    return 1;
+  }
   return 0;
 }
 

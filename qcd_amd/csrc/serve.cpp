@@ -31,17 +31,28 @@ size_t vec_bytes(long V, int prec) {
   return (size_t)st * 12 * spinor_comp_bytes(prec) + (prec == QCD_HIP_F16 ? (size_t)st * 4 : 0);
 }
 
-// device bytes a request holds while it runs
+// the solver parameters a request is served with: one batch of all its rhs,
+// even-odd preconditioned inner solve when every extent is even
+void request_params(const qcd_hip_request &q, qcd_hip_solver_params *p) {
+  qcd_hip_solver_default_params(p);
+  p->n_restart = q.n_restart;
+  p->tol = q.tol;
+  p->inner_prec = q.inner_prec;
+  p->batch = q.nrhs;
+  p->max_outer = 20000;
+  p->eo = (q.L[0] % 2 == 0 && q.L[1] % 2 == 0 && q.L[2] % 2 == 0 && q.L[3] % 2 == 0) ? 1 : 0;
+}
+
+// device bytes a request holds while it runs: its gauge field, x and b, and
+// the solver workspace exactly as qcd_hip_solver_create sizes it
 size_t request_bytes(const qcd_hip_request &r) {
   const long V = (long)r.L[0] * r.L[1] * r.L[2] * r.L[3];
   const long st = round_up(V, kTile);
-  const int n = r.n_restart, B = r.nrhs;
-  size_t b = (size_t)st * 36 * 16;                                    // FP64 gauge field
-  if (r.inner_prec != QCD_HIP_F64) b += (size_t)st * 36 * spinor_comp_bytes(r.inner_prec);
-  b += 2 * (size_t)B * vec_bytes(V, QCD_HIP_F64);                      // x, b
-  b += (size_t)(2 * n + 1) * B * vec_bytes(V, r.inner_prec);           // Krylov basis + correction
-  b += 2 * (size_t)B * vec_bytes(V, QCD_HIP_F64);                      // residual, system rhs
-  b += (size_t)B * (2 * n + 4) * ((V + kBlock - 1) / kBlock + 64) * 8; // reduction partials
+  qcd_hip_solver_params p;
+  request_params(r, &p);
+  size_t b = (size_t)st * 36 * 16;                                     // FP64 gauge field
+  b += 2 * (size_t)r.nrhs * vec_bytes(V, QCD_HIP_F64);                  // x, b
+  b += solver_device_bytes(r.L, &p);
   return b;
 }
 
@@ -58,13 +69,7 @@ int run_request(const qcd_hip_request &q, qcd_hip_result *res, double *solution,
   for (int r = 0; !rc && r < q.nrhs; r++) rc = point_source_device(b, r, 0, r / 3, r % 3, ws);
   if (!rc && hipStreamSynchronize(ws) != hipSuccess) rc = 2;
   qcd_hip_solver_params p;
-  qcd_hip_solver_default_params(&p);
-  p.n_restart = q.n_restart;
-  p.tol = q.tol;
-  p.inner_prec = q.inner_prec;
-  p.batch = q.nrhs;
-  p.max_outer = 20000;
-  p.eo = lat && eo_supported(lat) ? 1 : 0;   // even-odd preconditioned inner solve when the extents allow it
+  request_params(q, &p);
   if (!rc) rc = qcd_hip_solver_create(&S, lat, &p);
   if (!rc) rc = qcd_hip_solver_set_gauge(S, g);
   qcd_hip_solver_stats st;

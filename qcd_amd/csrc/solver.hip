@@ -614,6 +614,44 @@ void qcd_hip_solver_default_params(qcd_hip_solver_params *p) {
   p->graph = 1;
 }
 
+}  // extern "C"
+
+namespace qcdhip {
+// Device bytes qcd_hip_solver_create and qcd_hip_solver_set_gauge allocate
+// for a lattice of extents L with parameters prm: the same field shapes and
+// reduction rows, without allocating anything.
+size_t solver_device_bytes(const int L[4], const qcd_hip_solver_params *prm) {
+  qcd_hip_lattice_s lat{};
+  for (int d = 0; d < 4; d++) lat.L[d] = L[d];
+  lat.V = (long)L[0] * L[1] * L[2] * L[3];
+  lat.stride = round_up(lat.V, kTile);
+  auto field = [](long stride, int prec, int nrhs) {
+    return (size_t)12 * nrhs * stride * spinor_comp_bytes(prec) + (prec == QCD_HIP_F16 ? (size_t)4 * nrhs * stride : 0);
+  };
+  const int n = prm->n_restart, B = prm->batch, ip = prm->inner_prec;
+  const long V = lat.V;
+  const long Vi = prm->eo ? V / 2 : V;
+  const long sti = round_up(Vi, kTile);
+  size_t b = 0;
+  b += 2 * field(sti, ip, n * B) + field(sti, ip, B);                  // Rs, Ws, E
+  if (prm->eo) b += field(sti, ip, B) + 4 * field(sti, QCD_HIP_F64, B); // te, re64, ro64, t64, eo64
+  b += 2 * field(lat.stride, QCD_HIP_F64, B);                          // r64, bq
+  b += sizeof(GcrScal) * B + sizeof(double) * B + sizeof(int) * B;      // sc, rn, ids
+  const size_t nb = (size_t)grid_for(V);
+  const size_t rowmax = ((size_t)2 * (n + kDotChunk) + 4) * (nb + 64);
+  b += sizeof(double) * std::max(rowmax, (size_t)dslash_grid_bound(&lat)) * B;   // partials
+  if (ip != QCD_HIP_F64) b += (size_t)36 * lat.stride * spinor_comp_bytes(ip);   // inner-precision links
+  return b;
+}
+}  // namespace qcdhip
+
+extern "C" {
+
+size_t qcd_hip_solver_bytes(const int L[4], const qcd_hip_solver_params *prm) {
+  if (!L || !prm || prm->n_restart < 1 || prm->batch < 1 || L[0] < 1 || L[1] < 1 || L[2] < 1 || L[3] < 1) return 0;
+  return solver_device_bytes(L, prm);
+}
+
 int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hip_solver_params *prm) {
   *out = nullptr;
   QH_TRY(ensure_init());

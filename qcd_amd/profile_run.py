@@ -5,6 +5,11 @@ Runs `iters` Wilson Dslash applications and `iters` plain FP64 copies
 (qcd_hip_lincomb with a = 1, b = 0: 192 B read + 192 B written per site) on
 the same lattice, so that the counters of the copy calibrate those of Dslash.
 
+       python -m qcd_amd.profile_run --lattice 48 24 24 24 --prec f16 --rhs 48 [--schur]
+Runs `iters` batched launches over 48 right-hand sides instead (with --schur:
+the even-odd Schur operator M - H_oe M^-1 H_eo on odd-site half vectors, two
+hop launches per application).
+
        python -m qcd_amd.profile_run --solve --lattice 48 24 24 24 --prec f16 --rhs 48 --eo 1
 Runs one propagator solve instead: 12 point sources (spin x colour) at each
 of rhs/12 sites, kappa 0.132, eps 0.3, GCR(n_restart), tol 1e-10, all rhs in one
@@ -30,14 +35,19 @@ def main(argv=None):
     ap.add_argument("--occ", type=int, default=0, help="occupancy variant when --map is given")
     ap.add_argument("--sweep", action="store_true", help="time every candidate (QCD_HIP_TUNE_VERBOSE)")
     ap.add_argument("--solve", action="store_true", help="profile one batched propagator solve instead")
-    ap.add_argument("--rhs", type=int, default=12, help="--solve: right-hand sides (a multiple of 12)")
+    ap.add_argument("--rhs", type=int, default=None,
+                    help="right-hand sides: --solve a multiple of 12 (default 12); otherwise one batched launch")
+    ap.add_argument("--schur", action="store_true", help="profile the even-odd Schur operator instead of D")
     ap.add_argument("--eo", type=int, default=1, help="--solve: even-odd preconditioning")
     ap.add_argument("--n-restart", type=int, default=32, help="--solve: GCR cycle length (invert.ini: 32)")
     args = ap.parse_args(argv)
     L = tuple(args.lattice)
     Q.init(0)
     if args.solve:
-        return solve(Q, L, args.prec, args.rhs, args.eo, args.n_restart)
+        return solve(Q, L, args.prec, args.rhs or 12, args.eo, args.n_restart)
+    if args.schur:
+        return schur(Q, L, args.prec, args.rhs or 1, args.iters)
+    nrhs = args.rhs or 1
     lat = Q.Lattice(L)
     u = Q.gauge_generate(L, 0.3, 4001)
     psi = Q.spinor_gaussian(L, 4002)
@@ -47,8 +57,10 @@ def main(argv=None):
     if p != Q.F64:
         g = Q.Gauge(lat, p)
         Q.convert(g, g64)
-    x = Q.Spinor(lat, p).upload(psi)
-    y = Q.Spinor(lat, p)
+    x = Q.Spinor(lat, p, nrhs=nrhs)
+    for r in range(nrhs):
+        x.upload(psi if r == 0 else Q.spinor_gaussian(L, 4002 + r), rhs=r)
+    y = Q.Spinor(lat, p, nrhs=nrhs)
     m = 1 / (2 * 0.132) - 4
     if args.sweep:
         os.environ["QCD_HIP_TUNE_VERBOSE"] = "1"
@@ -62,8 +74,44 @@ def main(argv=None):
         Q.lincomb(b, 1.0, a)
     Q.synchronize()
     V = int(np.prod(L))
-    print(f"lattice {L} V={V} prec={args.prec} dslash mean {ms*1e3:.2f} us; copy bytes/launch {V*384}",
-          flush=True)
+    print(f"lattice {L} V={V} prec={args.prec} rhs={nrhs} dslash mean {ms*1e3:.2f} us "
+          f"({ms*1e3/nrhs:.2f} us per rhs); copy bytes/launch {V*384}", flush=True)
+    return 0
+
+
+def schur(Q, L, prec, nrhs, iters):
+    """iters applications of the even-odd Schur operator to nrhs odd-site half vectors."""
+    import time
+    T, X, Y, Z = L
+    lat = Q.Lattice(L)
+    half = Q.Lattice((T // 2, X, Y, Z))
+    g64 = Q.Gauge(lat)
+    Q.gauge_generate_device(g64, 0.3, 4001)
+    p = Q.PREC[prec]
+    g = g64
+    if p != Q.F64:
+        g = Q.Gauge(lat, p)
+        Q.convert(g, g64)
+    Vh = half.V
+    x = Q.Spinor(half, p, nrhs=nrhs)
+    for r in range(nrhs):
+        x.upload(Q.spinor_gaussian((Vh, 1, 1, 1), 4100 + r), rhs=r)
+    y = Q.Spinor(half, p, nrhs=nrhs)
+    m = 1 / (2 * 0.132) - 4
+    Q.dslash_schur(y, x, g, m)   # tunes and warms up
+    Q.synchronize()
+    t = time.perf_counter()
+    for _ in range(iters):
+        Q.dslash_schur(y, x, g, m)
+    Q.synchronize()
+    el = (time.perf_counter() - t) / iters
+    a = Q.Spinor(lat).upload(Q.spinor_gaussian(L, 4002))   # the plain FP64 copy that calibrates the counters
+    b = Q.Spinor(lat)
+    for _ in range(iters):
+        Q.lincomb(b, 1.0, a)
+    Q.synchronize()
+    print(f"lattice {L} prec={prec} rhs={nrhs} Schur operator {el*1e3:.3f} ms per application "
+          f"({el*1e6/nrhs:.2f} us per rhs)", flush=True)
     return 0
 
 
