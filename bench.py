@@ -39,11 +39,31 @@ def log(msg):
 
 
 def cpu_threads():
+    """Threads for the CPU baseline: every core this process may run on, at
+    most 16 (the host CPU share of one GPU on the benchmark machines)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def host_info():
+    """Core counts and CPU model of the host the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = None
+    return {"nproc": os.cpu_count(), "cores_allowed": allowed, "cpu_model": model}
 
 
 def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
@@ -67,7 +87,8 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
     return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
             "config": {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march over LDS-staged row tiles",
-                                      "z-march, forward z neighbour carried in LDS"][cfg[0] & 0xff],
+                                      "z-march, forward z neighbour carried in LDS", "z-march, two rhs per lane",
+                                      "matrix cores (MFMA), 16 sites x 4 rhs per wavefront"][cfg[0] & 0xff],
                        "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
                        "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}}
 
@@ -148,9 +169,13 @@ def pmc_traffic(L, prec, cfg, timeout=150):
 
 
 def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0.3, seed=2002,
-               sites=((0, 0, 0, 0),), eo=0):
+               sites=((0, 0, 0, 0),), eo=0, keep=None, ref=None):
     """Propagators of 12 point sources (spin x colour) at each site, every rhs
-    solved to ||b - D x|| / ||b|| <= 1e-10 and checked in FP64 afterwards."""
+    solved to ||b - D x|| / ||b|| <= 1e-10 and checked in FP64 afterwards.
+    keep (a dict) receives the solution field, the gauge field and the host
+    copy of rhs 0's solution, for the oracle re-check of the CPU leg; ref (a
+    kept dict of an FP64 leg on the same problem) gives the largest
+    ||x - x_FP64|| / ||x_FP64|| over rhs."""
     lat = Q.Lattice(L)
     g = Q.Gauge(lat)
     Q.gauge_generate_device(g, eps, seed)
@@ -172,9 +197,19 @@ def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0
     el = time.perf_counter() - t
     worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(nr))
     assert worst <= 1e-10, f"{prec} solve not converged: {worst}"
-    return {"solves_per_s": nr / el, "seconds": el, "rhs": nr, "batch": batch, "n_restart": n_restart,
-            "even_odd": bool(eo), "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"],
-            "worst_b_minus_Dx": worst}
+    out = {"solves_per_s": nr / el, "seconds": el, "rhs": nr, "batch": batch, "n_restart": n_restart,
+           "even_odd": bool(eo), "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"],
+           "worst_b_minus_Dx": worst}
+    if ref is not None:
+        d = Q.Spinor(lat)
+        err = 0.0
+        for r in range(nr):
+            Q.lincomb(d, 1.0, x, -1.0, ref["x"], rx=r, ry=r)
+            err = max(err, (d.norm2() / ref["x"].norm2(r)) ** 0.5)
+        out["max_rel_err_vs_fp64_solution"] = err
+    if keep is not None:
+        keep.update({"x": x, "g": g, "L": L, "kappa": kappa, "site": sites[0], "x0": x.download(0), "out": out})
+    return out
 
 
 def source_sites_24c48(L=(48, 24, 24, 24), seed=3003):
@@ -183,7 +218,7 @@ def source_sites_24c48(L=(48, 24, 24, 24), seed=3003):
     return [tuple(int(rng.integers(0, L[d])) for d in range(4)) for _ in range(4)]
 
 
-def pipeline_48c96_leg(Q, batch=4, seed=5005):
+def pipeline_48c96_leg(Q, batch=4, seed=5005, keep=None):
     """48^3x96, eps 0.3, kappa 0.132: 10 APE steps (alpha 0.5) of the links, 12
     point sources at the origin Gauss-smeared (alpha 4.0, 50 iterations, FP64)
     with the APE links, then a 12-rhs mixed solve (FP16 inner, even-odd) with
@@ -225,6 +260,9 @@ def pipeline_48c96_leg(Q, batch=4, seed=5005):
         assert worst <= 1e-10, f"48^3x96 solve not converged: {worst}"
         solves[n] = {"solve_seconds": t_solve, "solves_per_s": 12 / t_solve, "gcr_iterations": st["inner_iters"],
                      "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
+    if keep is not None:   # rhs 0 (a smeared source) for the oracle re-check
+        keep.update({"u": g.download(), "L": L, "kappa": kappa, "x0": x.download(0), "b0": b.download(0),
+                     "out": solves[12]})
     out = {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
                      "Gauss-smeared (alpha 4.0, 50 iterations, FP64), mixed FP16-inner even-odd solve, tol 1e-10",
            "ape_seconds": t_ape, "smear_seconds": t_smear, "batch": batch, **solves[32],
@@ -279,20 +317,42 @@ def cpu_baseline_leg(L, seconds=10.0):
                       f"{L[1]}^3x{L[0]} in {el:.1f} s, 960 B/site; same workload as the headline"}
 
 
-def cpu_solver_sample(cycles_per_rhs, n_restart=32):
-    """One GCR(32) cycle of the oracle (invert.c restated, sequential BLAS as
-    qcd_blas.c) at 16^3x32, scaled to solves/s with the GPU's cycle count."""
+def cpu_solve_leg(n_restart=32):
+    """One full FP64 propagator-column solve by the CPU oracle (invert.c's GCR
+    plus outer refinement, restated in C, BLAS reduced as on 64 MPI ranks) at
+    16^3x32, kappa 0.130, eps 0.3: the point source at the origin (spin 0,
+    colour 0) to ||b - D x|| / ||b|| <= 1e-10, timed end to end."""
     import oracle as O
     L = (32, 16, 16, 16)
-    O.set_threads(cpu_threads())
+    threads = cpu_threads()
+    O.set_threads(threads)
     u = O.gauge_generate(L, 0.3, 2002)
     b = O.point_source(L, (0, 0, 0, 0), 0, 0)
     geo = O.Geometry(L)
     t = time.perf_counter()
-    O.gcr(geo, b, u, 0.130, 0.0, n=n_restart, op=0, keep_q1=True)
+    rc, x, outer, rel = O.invert(geo, b, u, 0.130, 0.0, n=n_restart, op=0, keep_q1=False, max_res=1e-10,
+                                 max_outer=400)
     el = time.perf_counter() - t
-    return {"solves_per_s": 1.0 / (el * cycles_per_rhs), "seconds_per_gcr_cycle": el,
-            "cycles_per_rhs_assumed": cycles_per_rhs}
+    return {"solves_per_s": 1.0 / el, "seconds": el, "rc": rc, "outer_cycles": outer, "relres": rel,
+            "threads": threads, "n_restart": n_restart,
+            "sample": "one rhs of config 2 (16^3x32, 12 origin point sources) solved in full by the oracle"}
+
+
+def oracle_recheck(kept):
+    """The oracle's own b_minus_Dx residual (b_minus_Dx.c:152-172) of rhs 0 of
+    each kept solve (the GPU's solution, the point source at the first site)."""
+    import oracle as O
+    O.set_threads(cpu_threads())
+    out = {}
+    for name, k in kept.items():
+        u = k["u"] if "u" in k else k["g"].download()
+        geo = O.Geometry(k["L"])
+        b = k["b0"] if "b0" in k else O.point_source(k["L"], k["site"], 0, 0)
+        res = O.b_minus_Dx(geo, k["x0"], b, u, k["kappa"])
+        k["out"]["oracle_b_minus_Dx_rhs0"] = res
+        out[name] = res
+        del u
+    return out
 
 
 def main():
@@ -309,6 +369,7 @@ def main():
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure the headline kernel's HBM traffic")
     args = ap.parse_args()
+    kept = {}   # solutions re-checked by the oracle in the CPU leg
     if args.gpus != 1:
         print(f"bench.py: this build runs one rank on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is a design note, not built)", file=sys.stderr)
@@ -350,7 +411,9 @@ def main():
                 ("mixed_f16_batched12", "f16", 12, 0), ("fp64_eo_one_at_a_time", "f64", 1, 1),
                 ("mixed_f32_eo_one_at_a_time", "f32", 1, 1), ("mixed_f16_eo_batched12", "f16", 12, 1)]
         for name, prec, batch, eo in legs:
-            solves[name] = solves_leg(Q, prec, batch, 32, eo=eo)
+            kept[name] = {}
+            solves[name] = solves_leg(Q, prec, batch, 32, eo=eo, keep=kept[name],
+                                      ref=None if prec == "f64" else kept["fp64_one_at_a_time"])
             log(f"solves {name}: {solves[name]['solves_per_s']:.2f}/s ({solves[name]['gcr_iterations']} GCR iterations)")
         base = solves["fp64_one_at_a_time"]["solves_per_s"]
         base_eo = solves["fp64_eo_one_at_a_time"]["solves_per_s"]
@@ -361,7 +424,9 @@ def main():
         # invert.ini:8 sets N_restart = 32; a shorter cycle (12) does about as many GCR iterations here and halves
         # the orthogonalisation traffic, so the same two even-odd legs are also timed at 12
         for name, prec, batch in (("fp64_eo_one_at_a_time_n12", "f64", 1), ("mixed_f16_eo_batched12_n12", "f16", 12)):
-            solves[name] = solves_leg(Q, prec, batch, 12, eo=1)
+            kept[name] = {}
+            solves[name] = solves_leg(Q, prec, batch, 12, eo=1, keep=kept[name],
+                                      ref=None if prec == "f64" else kept["fp64_one_at_a_time"])
             log(f"solves {name}: {solves[name]['solves_per_s']:.2f}/s ({solves[name]['gcr_iterations']} GCR iterations)")
         solves["speedup_mixed_f16_eo_batched_vs_fp64_eo_n12"] = (solves["mixed_f16_eo_batched12_n12"]["solves_per_s"] /
                                                                  solves["fp64_eo_one_at_a_time_n12"]["solves_per_s"])
@@ -372,16 +437,23 @@ def main():
         for name, prec, batch, n in (("fp64_eo_one_at_a_time", "f64", 1, 32), ("mixed_f16_eo_batched48", "f16", 48, 32),
                                      ("fp64_eo_one_at_a_time_n12", "f64", 1, 12),
                                      ("mixed_f16_eo_batched48_n12", "f16", 48, 12)):
-            solves["multi_source_24c48"][name] = solves_leg(Q, prec, batch, n, L=L3, kappa=0.132, eps=0.3, seed=3003, sites=c3,
-                                                 eo=1)
+            kname = "c3_" + name
+            kept[kname] = {}
+            solves["multi_source_24c48"][name] = solves_leg(
+                Q, prec, batch, n, L=L3, kappa=0.132, eps=0.3, seed=3003, sites=c3, eo=1, keep=kept[kname],
+                ref=None if prec == "f64" else kept["c3_fp64_eo_one_at_a_time"])
             log(f"solves 24^3x48 48 rhs {name}: {solves['multi_source_24c48'][name]['solves_per_s']:.2f}/s")
         solves["multi_source_24c48"]["speedup_mixed_vs_fp64"] = (solves["multi_source_24c48"]["mixed_f16_eo_batched48"]["solves_per_s"] /
                                                       solves["multi_source_24c48"]["fp64_eo_one_at_a_time"]["solves_per_s"])
         solves["multi_source_24c48"]["speedup_mixed_vs_fp64_n12"] = (
             solves["multi_source_24c48"]["mixed_f16_eo_batched48_n12"]["solves_per_s"] /
             solves["multi_source_24c48"]["fp64_eo_one_at_a_time_n12"]["solves_per_s"])
+        for k in kept.values():   # the oracle re-check needs host data only; free the device fields
+            k["u"] = k.pop("g").download()
+            k.pop("x")
         if not args.no_pipeline48:
-            solves["pipeline_48c96"] = pipeline_48c96_leg(Q, batch=6)
+            kept["pipeline_48c96_gcr12"] = {}
+            solves["pipeline_48c96"] = pipeline_48c96_leg(Q, batch=6, keep=kept["pipeline_48c96_gcr12"])
             c5 = solves["pipeline_48c96"]
             log(f"48^3x96 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
                 f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s); GCR(12) solve "
@@ -404,12 +476,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        os.environ.setdefault("OMP_PROC_BIND", "close")   # one pinned thread per core
+        os.environ.setdefault("OMP_PLACES", "cores")
         cpu = cpu_baseline_leg(L)
-        log(f"cpu baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})")
+        cpu["host"] = host_info()
+        log(f"cpu baseline: {cpu['value']:.2f} GB/s ({cpu['sample']}) on {cpu['host']}")
+        cpu["solve"] = cpu_solve_leg()
+        log(f"cpu solve: {cpu['solve']}")
         if solves:
-            cyc = solves["fp64_one_at_a_time"]["outer_cycles"] / 12.0
-            cpu["solver_sample"] = cpu_solver_sample(cyc)
-            log(f"cpu solver sample: {cpu['solver_sample']}")
+            solves["cpu_fp64_solve_per_s"] = cpu["solve"]["solves_per_s"]
+            solves["oracle_recheck_rhs0"] = oracle_recheck(kept)
+            log(f"oracle re-check of rhs 0: {solves['oracle_recheck_rhs0']}")
+            assert max(solves["oracle_recheck_rhs0"].values()) <= 1e-10
 
     V = int(np.prod(L))
     out = {

@@ -71,6 +71,21 @@ static inline void qcd_mat3_mul_(qcd_complex_16 a[3][3], qcd_complex_16 b[3][3],
 }
 
 /* ---- vector BLAS (qcd_blas.h:242-267; lib/src/qcd_blas.c:137-491) ---- */
+/* 3x3 complex matrix helpers (include/qcd_blas.h:224-241, lib/src/qcd_blas.c:19-135):
+ * host code; addAdjoint = a + b^+, mulAdjoint = a b^+ */
+void qcd_zero3x3(qcd_complex_16 a[3][3]);
+void qcd_unit3x3(qcd_complex_16 a[3][3]);
+void qcd_minusUnit3x3(qcd_complex_16 a[3][3]);
+void qcd_sub3x3(qcd_complex_16 difference[3][3], qcd_complex_16 minuend[3][3], qcd_complex_16 subtrahend[3][3]);
+void qcd_add3x3(qcd_complex_16 sum[3][3], qcd_complex_16 summand1[3][3], qcd_complex_16 summand2[3][3]);
+void qcd_addAdjoint3x3(qcd_complex_16 sum[3][3], qcd_complex_16 summand1[3][3], qcd_complex_16 summand2[3][3]);
+void qcd_mul3x3(qcd_complex_16 product[3][3], qcd_complex_16 factor1[3][3], qcd_complex_16 factor2[3][3]);
+void qcd_mulAdjoint3x3(qcd_complex_16 product[3][3], qcd_complex_16 factor1[3][3], qcd_complex_16 factor2[3][3]);
+void qcd_scale3x3(qcd_complex_16 a[3][3], qcd_real_8 r);
+void qcd_cScale3x3(qcd_complex_16 a[3][3], qcd_complex_16 c);
+qcd_complex_16 qcd_trace3x3(qcd_complex_16 a[3][3]);
+void qcd_copy3x3(qcd_complex_16 dest[3][3], qcd_complex_16 src[3][3]);
+void qcd_dagger3x3(qcd_complex_16 a[3][3]);
 void qcd_zeroVector(qcd_vector *vec);
 void qcd_zeroPropagator(qcd_propagator *prop);
 void qcd_zeroGaugeField(qcd_gaugeField *u);
@@ -98,6 +113,24 @@ qcd_real_8 qcd_normVector(qcd_vector *vec);
  * exp(-i arg(det)/3) (include/qcd_blas.h:271, lib/src/qcd_blas.c:940-1364);
  * runs on the GPU */
 void qcd_projectSU33d(qcd_gaugeField *gf);
+/* every link projected onto SU(3) on the device (lib/src/qcd_blas.c:541-939);
+ * one matrix on the host (:1376-1769): the unitary polar factor times
+ * exp(-i arg(det) / 3), reached by Newton's iteration */
+void qcd_projectSU3(qcd_gaugeField *gf);
+void qcd_projectSU33x3(qcd_complex_16 g[3][3]);
+/* propagator and gauge-field arithmetic (lib/src/qcd_blas.c:192-200, :261-285,
+ * :358-390, :495-530), host code.  normGaugeField is the sum over links of
+ * |tr U|.  randomGaugeField fills links with drand48() entries (re, then im)
+ * and projects them onto SU(3) (:1771-1782). */
+void qcd_copyPropagatorPropagator(qcd_propagator *prop_new, qcd_propagator *prop_old);
+void qcd_mulPropagatorC(qcd_propagator *prop, qcd_complex_16 c);
+void qcd_mulPropagatorC3d(qcd_propagator *prop, qcd_complex_16 c, qcd_uint_4 t);
+void qcd_conjPropagator(qcd_propagator *prop);
+void qcd_subGaugeField(qcd_gaugeField *difference, qcd_gaugeField *minuend, qcd_gaugeField *subtrahend);
+void qcd_addGaugeField(qcd_gaugeField *sum, qcd_gaugeField *summand1, qcd_gaugeField *summand2);
+void qcd_scaleGaugeField(qcd_gaugeField *u, qcd_real_8 alpha);
+qcd_real_8 qcd_normGaugeField(qcd_gaugeField *u);
+void qcd_randomGaugeField(qcd_gaugeField *u);
 
 #ifdef __cplusplus
 }

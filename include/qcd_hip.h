@@ -59,6 +59,13 @@ int  qcd_hip_spinor_download(qcd_hip_field f, int rhs, double *aos);
 /* gauge <-> host double[V][4][3][3][2] (qcd_gaugeField.D) */
 int  qcd_hip_gauge_upload(qcd_hip_field f, const double *aos);
 int  qcd_hip_gauge_download(qcd_hip_field f, double *aos);
+/* host propagator double[V][4 mu][4 nu][3 c1][3 c2][2] (qcd_propagator.D,
+ * include/qcd.h:110-117) <-> 12 rhs of a spinor field: rhs r0 + 3 nu + c2
+ * holds the column (nu, c2), i.e. the spinor psi[mu][c1] = P[mu][nu][c1][c2]
+ * of qcd_copyVectorPropagator (lib/src/qcd_blas.c:182-212), for all 12
+ * columns in one pass on the device. */
+int  qcd_hip_propagator_upload(qcd_hip_field f, int r0, const double *prop);
+int  qcd_hip_propagator_download(qcd_hip_field f, int r0, double *prop);
 /* copy with precision change; same lattice, same kind (and nrhs for spinors) */
 int  qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src);
 
@@ -98,7 +105,9 @@ int qcd_hip_dslash_time(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge
  * same over tiles of whole t-rows staged in LDS with their x halo rows,
  * 4 | (n << 8) = traversal 2 with the forward z neighbour kept in LDS as the
  * next step's centre, 5 | (n << 8) = traversal 2 with two rhs per lane for
- * FP32/FP16 batches with the D or Q epilogue) and occupancy variant (0 = compiler default, 2 or 3 =
+ * FP32/FP16 batches with the D or Q epilogue, 6 = FP16 batches on the
+ * matrix cores: MFMA applies each link to the spin-projected neighbour
+ * spinors of 4 rhs per wavefront of 16 sites) and occupancy variant (0 = compiler default, 2 or 3 =
  * at least that many waves per SIMD, 16 = scheduling fences between hop
  * directions) on these fields and keep the fastest for their (lattice
  * extents, precisions).  Runs by itself on first use unless the environment
@@ -196,6 +205,11 @@ int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nst
  * temporal links copied) or 4; nsteps = 0 copies.  exp(iQ) by the
  * Peardon-Morningstar formula (qcd_smearing.c:293-370), exact at Q = 0. */
 int qcd_hip_stout_smear(qcd_hip_field out, qcd_hip_field in, double rho, int ndim, int nsteps);
+
+/* every link of an FP64 gauge field projected onto SU(3) in place
+ * (lib/src/qcd_blas.c:541-939; the unitary polar factor times
+ * exp(-i arg(det) / 3)) */
+int qcd_hip_project_su3(qcd_hip_field gauge64);
 
 /* mean plaquette of an FP64 gauge field (lib/src/qcd_observables.c:15-36) */
 int qcd_hip_plaquette(qcd_hip_field gauge64, double *plaq);

@@ -173,6 +173,16 @@ static inline void qcd_phase_mul_(double *x, int n, double pr, double pi)
 #define qcd_PHASE_MUL(x,t,pr,pi)             do { (t) = 0; qcd_phase_mul_((x),12,(pr),(pi)); } while (0)
 #define qcd_PHASE_MUL_HALFSPINOR(x,t,pr,pi)  do { (t) = 0; qcd_phase_mul_((x),6,(pr),(pi)); } while (0)
 
+/* Per-site helpers on one 4 x 3 spinor (include/qcd_wilson.h:627-632,
+ * lib/src/qcd_wilson.c:18-103), host code: (1 +- gamma_mu) vec, U vec and
+ * U^+ vec on all four spins, (mp4 + i tm gamma5) vec, and vec * phi. */
+void qcd_onePlusGamma(qcd_complex_16 result[][3], qcd_complex_16 vec[][3], qcd_uint_2 mu);
+void qcd_oneMinusGamma(qcd_complex_16 result[][3], qcd_complex_16 vec[][3], qcd_uint_2 mu);
+void qcd_applyU(qcd_complex_16 result[][3], qcd_complex_16 u[][3], qcd_complex_16 vec[][3]);
+void qcd_applyUDagger(qcd_complex_16 result[][3], qcd_complex_16 u[][3], qcd_complex_16 vec[][3]);
+void qcd_massTerms(qcd_complex_16 result[][3], qcd_complex_16 vec[][3], qcd_real_8 mp4, qcd_real_8 tm);
+void qcd_phaseMul(qcd_complex_16 vec[][3], qcd_complex_16 phi);
+
 /* Operators (include/qcd_wilson.h:634-637).  All three D spellings compute
  * the same operator; qcd_applyQTMOp computes Q = i gamma5 D
  * (lib/src/qcd_wilson.c:795-814).  Return 0 on success, 1 on invalid input
@@ -182,6 +192,17 @@ int qcd_applyWilsonTMOp(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_re
 int qcd_applyWilsonTMOpNoScripts(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_real_8 m, qcd_real_8 tm);
 int qcd_applyWilsonTMOpNoOptim(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_real_8 m, qcd_real_8 tm);
 int qcd_applyQTMOp(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_real_8 m, qcd_real_8 tm);
+
+/* Extensions without a reference counterpart: D (or Q) applied to all 12
+ * columns (nu, c2) of a propagator in one batched device launch, the same
+ * result as qcd_copyVectorPropagator + qcd_applyWilsonTMOp (or
+ * qcd_applyQTMOp) + qcd_copyPropagatorVector per column
+ * (lib/src/qcd_blas.c:182-212).  Return 0 on success, 1 on invalid input
+ * (uninitialised fields, mismatched geometry, Dp aliasing p). */
+int qcd_hip_applyWilsonTMOpPropagator(qcd_propagator *Dp, qcd_propagator *p, qcd_gaugeField *u, qcd_real_8 m,
+                                      qcd_real_8 tm);
+int qcd_hip_applyQTMOpPropagator(qcd_propagator *Dp, qcd_propagator *p, qcd_gaugeField *u, qcd_real_8 m,
+                                 qcd_real_8 tm);
 
 /* Extension without a reference counterpart: keep u resident in HBM so that
  * the drop-in operators, smearing and BLAS above skip its upload.  Library

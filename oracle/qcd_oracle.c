@@ -371,51 +371,97 @@ void orc_gauss3d_all(const orc_geo *g, double *v, const double *u, double alpha)
    free(v2);
 }
 
-/* ------------------------------------------------- vector BLAS, qcd_blas.c */
-void orc_scale(double *v, double r, long V) { long i; for (i = 0; i < 24 * V; i++) v[i] *= r; }   /* :301-308 */
-void orc_sub(double *d, const double *a, const double *b, long V) { long i; for (i = 0; i < 24 * V; i++) d[i] = a[i] - b[i]; } /* :345-354 */
-void orc_add(double *s, const double *a, const double *b, long V) { long i; for (i = 0; i < 24 * V; i++) s[i] = a[i] + b[i]; } /* :403-412 */
+/* ------------------------------------------------- vector BLAS, qcd_blas.c
+ * Elementwise loops run in parallel over sites.  Reductions follow the
+ * reference run on ORC_RANKS MPI ranks: each rank sums its own contiguous
+ * block of the vector sequentially (qcd_blas.c:456-491) and the rank sums are
+ * combined by MPI_Allreduce (:473, :488), here added in rank order.  The
+ * blocks are summed in parallel, and the result does not depend on the
+ * number of threads. */
+#define ORC_RANKS 64
+
+void orc_scale(double *v, double r, long V)   /* :301-308 */
+{
+   long i;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 24 * V; i++) v[i] *= r;
+}
+
+void orc_sub(double *d, const double *a, const double *b, long V)   /* :345-354 */
+{
+   long i;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 24 * V; i++) d[i] = a[i] - b[i];
+}
+
+void orc_add(double *s, const double *a, const double *b, long V)   /* :403-412 */
+{
+   long i;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 24 * V; i++) s[i] = a[i] + b[i];
+}
 
 /* out = a*x + y (:414-432); safe when out aliases y */
 void orc_axpy(double *out, double are, double aim, const double *x, const double *y, long V)
 {
    long i;
-   for (i = 0; i < 24 * V; i += 2) {
-      double xr = x[i], xi = x[i + 1];
-      out[i] = y[i] + xr * are - xi * aim;
-      out[i + 1] = y[i + 1] + xi * are + xr * aim;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 12 * V; i++) {
+      double xr = x[2*i], xi = x[2*i + 1];
+      out[2*i] = y[2*i] + xr * are - xi * aim;
+      out[2*i + 1] = y[2*i + 1] + xi * are + xr * aim;
    }
 }
 
 void orc_mulc(double *v, double are, double aim, long V)   /* :237-245 */
 {
    long i;
-   for (i = 0; i < 24 * V; i += 2) {
-      double re = v[i] * are - v[i + 1] * aim;
-      v[i + 1] = v[i] * aim + v[i + 1] * are;
-      v[i] = re;
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < 12 * V; i++) {
+      double re = v[2*i] * are - v[2*i + 1] * aim;
+      v[2*i + 1] = v[2*i] * aim + v[2*i + 1] * are;
+      v[2*i] = re;
    }
 }
 
-/* sum conj(a) b, sequential FP64 (:456-475) */
+/* sum conj(a) b (:456-475): per-rank sequential sums, then the rank sums */
 void orc_dot(double out[2], const double *a, const double *b, long V)
 {
-   long i;
-   double re = 0.0, im = 0.0;
-   for (i = 0; i < 12 * V; i++) {
-      re += a[2*i] * b[2*i] + a[2*i + 1] * b[2*i + 1];
-      im += a[2*i] * b[2*i + 1] - a[2*i + 1] * b[2*i];
+   double part[ORC_RANKS][2];
+   const long n = 12 * V;
+   int r;
+#pragma omp parallel for schedule(static)
+   for (r = 0; r < ORC_RANKS; r++) {
+      long i;
+      double re = 0.0, im = 0.0;
+      for (i = n * r / ORC_RANKS; i < n * (r + 1) / ORC_RANKS; i++) {
+         re += a[2*i] * b[2*i] + a[2*i + 1] * b[2*i + 1];
+         im += a[2*i] * b[2*i + 1] - a[2*i + 1] * b[2*i];
+      }
+      part[r][0] = re;
+      part[r][1] = im;
    }
-   out[0] = re;
-   out[1] = im;
+   out[0] = out[1] = 0.0;
+   for (r = 0; r < ORC_RANKS; r++) {
+      out[0] += part[r][0];
+      out[1] += part[r][1];
+   }
 }
 
-/* sqrt(sum x^2), sequential FP64 (:478-491) */
+/* sqrt(sum x^2) (:478-491): per-rank sequential sums, then the rank sums */
 double orc_norm(const double *v, long V)
 {
-   long i;
-   double s = 0.0;
-   for (i = 0; i < 24 * V; i++) s += v[i] * v[i];
+   double part[ORC_RANKS], s = 0.0;
+   const long n = 24 * V;
+   int r;
+#pragma omp parallel for schedule(static)
+   for (r = 0; r < ORC_RANKS; r++) {
+      long i;
+      double t = 0.0;
+      for (i = n * r / ORC_RANKS; i < n * (r + 1) / ORC_RANKS; i++) t += v[i] * v[i];
+      part[r] = t;
+   }
+   for (r = 0; r < ORC_RANKS; r++) s += part[r];
    return sqrt(s);
 }
 

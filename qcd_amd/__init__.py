@@ -165,6 +165,10 @@ def lib():
         "qcd_applyWilsonTMOpNoScripts": (i, [pf, pf, pf, d, d]),
         "qcd_applyWilsonTMOpNoOptim": (i, [pf, pf, pf, d, d]),
         "qcd_applyQTMOp": (i, [pf, pf, pf, d, d]),
+        "qcd_hip_applyWilsonTMOpPropagator": (i, [pf, pf, pf, d, d]),
+        "qcd_hip_applyQTMOpPropagator": (i, [pf, pf, pf, d, d]),
+        "qcd_hip_propagator_upload": (i, [_vp, i, _dp]),
+        "qcd_hip_propagator_download": (i, [_vp, i, _dp]),
         "qcd_hip_bindGaugeField": (i, [pf]),
         "qcd_hip_unbindGaugeField": (None, []),
         "qcd_gaussIteration3d": (i, [pf, pf, d, ctypes.c_uint32]),

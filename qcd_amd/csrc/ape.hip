@@ -370,11 +370,41 @@ int ape_step(qcd_hip_field out, qcd_hip_field in, double alpha, hipStream_t st) 
   return 0;
 }
 
+// every link (all four directions) projected onto SU(3) in place
+// (lib/src/qcd_blas.c:541-939): one lane per (site, direction)
+__global__ void __launch_bounds__(kBlock) k_project_links(GaugeIn<QCD_HIP_F64> U, double2 *__restrict__ data,
+                                                          unsigned V, size_t stride) {
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (s >= V) return;
+  const unsigned mu = blockIdx.y;
+  C m[9];
+  U.load(mu, s, m);
+  project_su3(m);
+  double2 *o = data + (size_t)mu * 9 * stride + (size_t)(s / kTile) * 9 * kTile + s % kTile;
+#pragma unroll
+  for (int k = 0; k < 9; k++) o[(size_t)k * kTile] = make_double2(m[k].re, m[k].im);
+}
 }  // namespace qcdhip
 
 using namespace qcdhip;
 
 extern "C" {
+
+int qcd_hip_project_su3(qcd_hip_field g) {
+  QH_TRY(ensure_init());
+  if (!g || g->kind != QCD_HIP_GAUGE || g->prec != QCD_HIP_F64) {
+    set_error("project_su3: expected an FP64 gauge field");
+    return 1;
+  }
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  hipStream_t st = ctx()->stream;
+  gauge_written(g);
+  k_project_links<<<dim3(grid_for(g->lat->V), 4), kBlock, 0, st>>>(GaugeIn<QCD_HIP_F64>::of(g), (double2 *)g->data,
+                                                                   (unsigned)g->lat->V, (size_t)g->lat->stride);
+  QH_CHECK(hipGetLastError());
+  QH_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
 
 int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nsteps) {
   QH_TRY(ensure_init());
@@ -389,6 +419,7 @@ int qcd_hip_ape_smear(qcd_hip_field out, qcd_hip_field in, double alpha, int nst
       return 1;
     }
   std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  gauge_written(out);
   hipStream_t st = ctx()->stream;
   if (nsteps == 0) {
     QH_CHECK(hipMemcpyAsync(out->data, in->data, out->bytes, hipMemcpyDeviceToDevice, st));
@@ -430,6 +461,7 @@ int qcd_hip_stout_smear(qcd_hip_field out, qcd_hip_field in, double rho, int ndi
       return 1;
     }
   std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  gauge_written(out);
   hipStream_t st = ctx()->stream;
   if (nsteps == 0) {
     QH_CHECK(hipMemcpyAsync(out->data, in->data, out->bytes, hipMemcpyDeviceToDevice, st));

@@ -160,11 +160,103 @@ int lincomb_launch(qcd_hip_field out, int ro, double are, double aim, qcd_hip_fi
   return 0;
 }
 
+// ------------------------------------------------------------ propagator columns
+// A host propagator is [V][4 mu][4 nu][3 c1][3 c2] complex (include/qcd.h:110-117).
+// Its column (nu, c2) is the spinor psi[mu][c1] = P[mu][nu][c1][c2]
+// (qcd_copyVectorPropagator, lib/src/qcd_blas.c:182-212); here column
+// 3 nu + c2 is rhs r0 + 3 nu + c2 of a spinor field.  Grid (sites, 12
+// columns): consecutive lanes take consecutive sites of one column, so the
+// tiled field is written (read) in whole segments; each site's 2304-byte
+// block of the propagator is read (written) by its 12 column lanes.
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_prop_gather(const double2 *__restrict__ prop, SpinorSet<P> out, int r0,
+                                                        unsigned V) {
+  using R = typename Prec<P>::R;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= V) return;
+  const int col = blockIdx.y, nu = col / 3, c2 = col % 3;
+  const double2 *p = prop + (size_t)s * 144 + nu * 9 + c2;
+  Cx<R> v[12];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+    for (int c1 = 0; c1 < 3; c1++) {
+      const double2 d = p[mu * 36 + c1 * 3];
+      v[3 * mu + c1] = Cx<R>{(R)d.x, (R)d.y};
+    }
+  out.store(r0 + col, s, v);
+}
+
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_prop_scatter(SpinorSet<P> in, int r0, double2 *__restrict__ prop,
+                                                         unsigned V) {
+  using R = typename Prec<P>::R;
+  const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= V) return;
+  const int col = blockIdx.y, nu = col / 3, c2 = col % 3;
+  Cx<R> v[12];
+  in.load(r0 + col, s, v);
+  double2 *p = prop + (size_t)s * 144 + nu * 9 + c2;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+    for (int c1 = 0; c1 < 3; c1++) p[mu * 36 + c1 * 3] = make_double2((double)v[3 * mu + c1].re, (double)v[3 * mu + c1].im);
+}
+
 }  // namespace qcdhip
 
 using namespace qcdhip;
 
 extern "C" {
+
+// propagator (host AoS) -> rhs r0 .. r0+11 of f (up = true), or back
+static int propagator_copy(qcd_hip_field f, int r0, double *prop, bool up) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // library stream
+  QH_TRY(ensure_init());
+  if (!f || f->kind != QCD_HIP_SPINOR || !prop || r0 < 0 || r0 + 12 > f->nrhs) {
+    set_error("propagator %s: need a spinor field with rhs r0 .. r0+11 and a host propagator",
+              up ? "upload" : "download");
+    return 1;
+  }
+  const unsigned V = (unsigned)f->lat->V;
+  const size_t bytes = (size_t)V * 144 * sizeof(double2);
+  double2 *buf = nullptr;
+  QH_TRY(dev_alloc((void **)&buf, bytes));
+  hipStream_t st = ctx()->stream;
+  const dim3 grid((unsigned)grid_for(V), 12);
+  int rc = 0;
+  if (up && hipMemcpyAsync(buf, prop, bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = 2;
+  if (!rc) {
+    switch (f->prec) {
+      case QCD_HIP_F64:
+        if (up) k_prop_gather<QCD_HIP_F64><<<grid, kBlock, 0, st>>>(buf, SpinorSet<QCD_HIP_F64>::of(f), r0, V);
+        else k_prop_scatter<QCD_HIP_F64><<<grid, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(f), r0, buf, V);
+        break;
+      case QCD_HIP_F32:
+        if (up) k_prop_gather<QCD_HIP_F32><<<grid, kBlock, 0, st>>>(buf, SpinorSet<QCD_HIP_F32>::of(f), r0, V);
+        else k_prop_scatter<QCD_HIP_F32><<<grid, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F32>::of(f), r0, buf, V);
+        break;
+      default:
+        if (up) k_prop_gather<QCD_HIP_F16><<<grid, kBlock, 0, st>>>(buf, SpinorSet<QCD_HIP_F16>::of(f), r0, V);
+        else k_prop_scatter<QCD_HIP_F16><<<grid, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F16>::of(f), r0, buf, V);
+        break;
+    }
+    if (hipGetLastError() != hipSuccess) rc = 2;
+  }
+  if (!rc && !up && hipMemcpyAsync(prop, buf, bytes, hipMemcpyDeviceToHost, st) != hipSuccess) rc = 2;
+  if (hipStreamSynchronize(st) != hipSuccess) rc = 2;
+  dev_free(buf, bytes);
+  if (rc) set_error("propagator %s: device error", up ? "upload" : "download");
+  return rc;
+}
+
+int qcd_hip_propagator_upload(qcd_hip_field f, int r0, const double *prop) {
+  return propagator_copy(f, r0, const_cast<double *>(prop), true);
+}
+
+int qcd_hip_propagator_download(qcd_hip_field f, int r0, double *prop) {
+  return propagator_copy(f, r0, prop, false);
+}
 
 int qcd_hip_norm2(qcd_hip_field v, int rhs, double *out) {
   std::lock_guard<std::recursive_mutex> lock(state_mutex());   // shared scratch and stream

@@ -315,6 +315,52 @@ int qcd_applyQTMOp(qcd_vector *Dv, qcd_vector *v, qcd_gaugeField *u, qcd_real_8 
   return apply_operator("qcd_applyQTMOp", QCD_HIP_OP_Q, Dv, v, u, m, tm);
 }
 
+// D (or Q) applied to all 12 columns of a propagator in one batched launch:
+// the columns are gathered on the device into 12 rhs of a library field
+// (scratch slot 7), the operator runs over the 12 rhs together, and the
+// results are scattered back into Dp (scratch slot 3).  The reference applies
+// the operator column by column after qcd_copyVectorPropagator
+// (lib/src/qcd_blas.c:182-212, lib/src/qcd_wilson.c:114-336).
+static int apply_propagator(const char *who, int op, qcd_propagator *Dp, qcd_propagator *p, qcd_gaugeField *u,
+                            double m, double tm) {
+  if (!p || !p->initialized || !Dp || !Dp->initialized || !u || !u->initialized) {
+    fprintf(stderr, "Error in %s! Propagators and gauge field must be properly initialized\n", who);
+    return 1;
+  }
+  const qcd_geometry *a = p->geo, *b = u->geo, *c = Dp->geo;
+  bool bad = !a || !b || !c;
+  for (int d = 0; d < 4 && !bad; d++) bad = a->L[d] != b->L[d] || a->L[d] != c->L[d] || a->lL[d] != b->lL[d];
+  if (bad) {
+    fprintf(stderr, "Error in %s! Geometry not initialized or missmatched\n", who);
+    return 1;
+  }
+  if (Dp->D == p->D) {
+    fprintf(stderr, "Error in %s! Output must not alias the input\n", who);
+    return 1;
+  }
+  qcd_hip_lattice lat;
+  qcd_hip_field g, in = nullptr, out = nullptr;
+  int rc = lattice_for(u->geo, &lat);
+  if (!rc) rc = gauge_for(u, lat, &g);
+  if (!rc) rc = scratch_spinor(7, lat, QCD_HIP_F64, 12, &in);
+  if (!rc) rc = scratch_spinor(3, lat, QCD_HIP_F64, 12, &out);
+  if (!rc && (in->nrhs != 12 || out->nrhs != 12)) rc = 1;   // the slots only ever hold 12-rhs fields
+  if (!rc) rc = qcd_hip_propagator_upload(in, 0, (const double *)p->D);
+  if (!rc) rc = qcd_hip_dslash(out, in, g, m, tm, op);
+  if (!rc) rc = qcd_hip_propagator_download(out, 0, (double *)Dp->D);
+  return report(who, rc);
+}
+
+int qcd_hip_applyWilsonTMOpPropagator(qcd_propagator *Dp, qcd_propagator *p, qcd_gaugeField *u, qcd_real_8 m,
+                                      qcd_real_8 tm) {
+  return apply_propagator("qcd_hip_applyWilsonTMOpPropagator", QCD_HIP_OP_D, Dp, p, u, m, tm);
+}
+
+int qcd_hip_applyQTMOpPropagator(qcd_propagator *Dp, qcd_propagator *p, qcd_gaugeField *u, qcd_real_8 m,
+                                 qcd_real_8 tm) {
+  return apply_propagator("qcd_hip_applyQTMOpPropagator", QCD_HIP_OP_Q, Dp, p, u, m, tm);
+}
+
 // Keep u resident in HBM: later drop-in calls with this field skip the
 // upload until qcd_hip_unbindGaugeField().  The caller must rebind after
 // changing u->D.
@@ -438,6 +484,19 @@ void qcd_projectSU33d(qcd_gaugeField *gf) {
   if (!rc) rc = dropin_gauge_written(gf->D);
   qcd_hip_field_destroy(out);
   report("qcd_projectSU33d", rc);
+}
+
+// lib/src/qcd_blas.c:541-939: every link (all four directions) projected
+// onto SU(3) in place, on the device
+void qcd_projectSU3(qcd_gaugeField *gf) {
+  if (!gf || !gf->initialized) return;
+  qcd_hip_lattice lat;
+  qcd_hip_field g;
+  int rc = lattice_for(gf->geo, &lat);
+  if (!rc) rc = gauge_for(gf, lat, &g);   // a bound field is projected in its resident copy
+  if (!rc) rc = qcd_hip_project_su3(g);
+  if (!rc) rc = qcd_hip_gauge_download(g, (double *)gf->D);
+  report("qcd_projectSU3", rc);
 }
 
 // lib/src/qcd_observables.c:15-36

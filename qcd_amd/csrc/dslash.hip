@@ -475,6 +475,14 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
                       qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
   // z-march traversals interleave the batch rows into grid.x (see block_row)
+  if (a.map == MAP_MMA) {
+    if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16 && !RED && (EPI == EPI_D || EPI == EPI_Q)) {
+      return dslash_mma_launch(EPI, a, out, ro, in, ri, g, b, rb, stream);
+    } else {
+      set_error("dslash: the matrix-core traversal needs FP16 spinors and links, D or Q and no reduction");
+      return 1;
+    }
+  }
   const bool zmarch = a.map == MAP_ZMARCH || a.map == MAP_ZTILE || a.map == MAP_ZMARCH_LDS || a.map == MAP_ZMARCH2;
   const dim3 gr(zmarch ? (unsigned)grid * a.count : (unsigned)grid, zmarch ? 1u : (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
@@ -564,6 +572,12 @@ DslashArgs make_args(qcd_hip_lattice lat, double m, double tm) {
 // Fill the traversal fields of a; returns the grid size.  For MAP_ZMARCH the
 // low byte of `map` is 2 and the z steps per lane are map >> 8.
 int set_traversal(DslashArgs &a, int map, int ps) {
+  if ((map & 0xff) == MAP_MMA) {
+    // 16-site blocks; the launcher derives its grid from them and the batch
+    a.map = MAP_MMA;
+    a.zlen = 1;
+    return (int)((a.V + 15) / 16);
+  }
   if ((map & 0xff) == MAP_ZTILE) {
     // rows per tile: the most whole t-rows in one workgroup, preferring a lane
     // count that fills whole wavefronts; falls back to MAP_ZMARCH when no tile
@@ -682,6 +696,20 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
     // the two-rhs traversal does not apply: same z-march, one rhs per lane
     a.map = MAP_ZMARCH;
   }
+  if (a.map == MAP_MMA && (in->prec != QCD_HIP_F16 || g->prec != QCD_HIP_F16 || red || (epi != EPI_D && epi != EPI_Q))) {
+    // the matrix-core traversal does not apply: the z-march of 4 steps
+    const int gz = set_traversal(a, MAP_ZMARCH | (4 << 8), in->prec);
+    a.nblk = (unsigned)gz;
+    if (grid_out) *grid_out = gz;
+    const int ps = in->prec, pg = g->prec;
+    if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
+      return launch_epi<QCD_HIP_F64, QCD_HIP_F64>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+    if (ps == QCD_HIP_F32 && pg == QCD_HIP_F32)
+      return launch_epi<QCD_HIP_F32, QCD_HIP_F32>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+    if (ps == QCD_HIP_F32 && pg == QCD_HIP_F16)
+      return launch_epi<QCD_HIP_F32, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+    return launch_epi<QCD_HIP_F16, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+  }
   if (grid_out) *grid_out = grid;
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
@@ -762,6 +790,18 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
     };
     QH_TRY(time_batch(best.map, best.occ, &bbest));
     const float one = bbest;
+    if (in->prec == QCD_HIP_F16 && g->prec == QCD_HIP_F16) {
+      float ms = 0.f;
+      QH_TRY(time_batch(MAP_MMA, 0, &ms));
+      if (getenv("QCD_HIP_TUNE_VERBOSE"))
+        fprintf(stderr, "tune L=%dx%dx%dx%d ps=%d pg=%d %d rhs matrix cores: %.2f us (one-rhs %.2f)\n", key.L[0],
+                key.L[1], key.L[2], key.L[3], key.ps, key.pg, cnt, ms * 1e3, one * 1e3);
+      if (ms < bbest) {
+        bbest = ms;
+        best.bmap = MAP_MMA;
+        best.bocc = 0;
+      }
+    }
     for (int z : {2, 3, 4, 6, 8})
       for (int occ : {0, 16}) {
         float ms = 0.f;
@@ -956,7 +996,8 @@ int qcd_hip_dslash_candidates(int *maps, int max) {
 int qcd_hip_dslash_set_config(qcd_hip_field in, qcd_hip_field gauge, int map, int occ) {
   QH_TRY(ensure_init());
   const int m0 = map & 0xff;
-  if (!in || !gauge || m0 < 0 || m0 > 5 || (m0 >= 2 && (map >> 8) < 1) || (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
+  if (!in || !gauge || m0 < 0 || m0 > 6 || (m0 >= 2 && m0 <= 5 && (map >> 8) < 1) ||
+      (occ != 0 && occ != 2 && occ != 3 && occ != 16)) {
     set_error("dslash_set_config: bad arguments");
     return 1;
   }

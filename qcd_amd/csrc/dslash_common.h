@@ -322,7 +322,11 @@ struct GaugeIn {
 //  MAP_ZMARCH2: MAP_ZMARCH for batches (FP32 and FP16): each lane works on
 //              two right-hand sides of its column and loads every link once
 //              for both.
-enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3, MAP_ZMARCH_LDS = 4, MAP_ZMARCH2 = 5 };
+//  MAP_MMA   : batches of FP16 spinors and links on the matrix cores: a
+//              wavefront owns 16 sites x 4 right-hand sides and applies each
+//              link to the projected neighbour spinors of its 4 rhs with
+//              MFMA (mma.hip).
+enum { MAP_XCD = 0, MAP_SLAB = 1, MAP_ZMARCH = 2, MAP_ZTILE = 3, MAP_ZMARCH_LDS = 4, MAP_ZMARCH2 = 5, MAP_MMA = 6 };
 
 struct DslashArgs {
   unsigned V;
@@ -754,6 +758,11 @@ __device__ __forceinline__ void block_reduce_store(double v, double *partials, u
     partials[slot] = t;
   }
 }
+
+// matrix-core batch launch (mma.hip): FP16 spinors and links, D or Q
+// epilogue, batch rows as in DslashArgs
+int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
+                      qcd_hip_field b, int rb, hipStream_t st);
 
 // launchers defined in dslash.hip
 // Batched launch over `count` right-hand sides (see DslashArgs::ids): row y

@@ -9,69 +9,9 @@
 // Half-lattice fields hold the sites of one parity: site l of parity p is
 // entry l >> 1 (every extent even, so the t-rows alternate parities).
 // The hopping sum uses the projector/link/phase arithmetic of the full Dslash.
-#include "blas_common.h"
+#include "eo_common.h"
 
 namespace qcdhip {
-
-// EO epilogues: out = M^-1 H in | M b - H in | b - H in | M^-1 (b - H in)
-enum { EO_MINV = 0, EO_SCHUR = 1, EO_SUB = 2, EO_SUB_MINV = 3 };
-
-struct EoArgs {
-  DslashArgs d;            // full-lattice geometry, phases, mass, batch
-  unsigned Vh;             // sites per parity
-  unsigned par;            // parity of the output sites
-  double mi_re, mi_im;     // 1 / (m + 4 + i tm)
-  unsigned rgroup;         // batch rows per group of the hop traversal (eo_row)
-};
-
-// Block decomposition of the batched hop: one 1D grid of nblk * count
-// workgroups, remapped per XCD.  The batch rows go in groups of rgroup; within
-// a group the row is the fastest index, so a group's rows read each site
-// block's links together through one XCD's L2, and successive site blocks of
-// the group follow, so the z-neighbour planes read a few site blocks later are
-// still in that L2.  rgroup = count shares the links most, rgroup = 1 keeps
-// one row's spinors most local.
-__device__ __forceinline__ BlockRow eo_row_of(unsigned count, unsigned R, unsigned nb) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  const unsigned full = count / R, span = nb * R;
-  if (lb < full * span) {
-    const unsigned g = lb / span, w = lb - g * span;
-    return BlockRow{g * R + w % R, w / R};
-  }
-  const unsigned rem = count - full * R, w = lb - full * span;
-  return BlockRow{full * R + w % rem, w / rem};
-}
-__device__ __forceinline__ BlockRow eo_row(const EoArgs &e) { return eo_row_of(e.d.count, e.rgroup, e.d.nblk); }
-
-// the site of parity par stored at half-lattice entry h
-__device__ __forceinline__ Site eo_site(unsigned h, unsigned par, const DslashArgs &a) {
-  Site st;
-  const unsigned l0 = 2 * h;
-  const unsigned r = l0 / a.L[0];
-  const unsigned t0 = l0 - r * a.L[0];
-  const unsigned r2 = r / a.L[1];
-  st.x = r - r2 * a.L[1];
-  st.z = r2 / a.L[2];
-  st.y = r2 - st.z * a.L[2];
-  const unsigned odd = (par + st.x + st.y + st.z) & 1u;
-  st.t = t0 + odd;
-  st.s = l0 + odd;
-  st.valid = true;
-  return st;
-}
-
-// M v (inv = false) or M^-1 v (inv = true) on one site: spins 0, 1 carry
-// m + 4 + i tm, spins 2, 3 its conjugate (g5 = diag(1, 1, -1, -1))
-template <bool INV, typename R>
-__device__ __forceinline__ void eo_mass(Cx<R> v[12], const EoArgs &e) {
-  const R are = INV ? (R)e.mi_re : (R)e.d.mp4;
-  const R aim = INV ? (R)e.mi_im : (R)e.d.tm;
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    const R im = k < 6 ? aim : -aim;
-    v[k] = Cx<R>{fma(are, v[k].re, -(im * v[k].im)), fma(are, v[k].im, im * v[k].re)};
-  }
-}
 
 // both hops of direction MU into acc, the neighbour spinors read from the
 // half-lattice field of the other parity
@@ -95,29 +35,6 @@ __device__ __forceinline__ void eo_order_point(Cx<R> acc[4][3]) {
 #pragma unroll
       for (int c = 0; c < 3; c++) asm volatile("" : "+v"(acc[q][c].re), "+v"(acc[q][c].im));
     __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-// o from H in = -1/2 acc by the epilogue EPI (b read at half-site h)
-template <int EPI, int PS, typename R>
-__device__ __forceinline__ void eo_epilogue(Cx<R> o[12], const Cx<R> acc[4][3], const EoArgs &e,
-                                            const SpinorIn<PS> &b, unsigned h) {
-  if constexpr (EPI == EO_MINV) {
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) o[3 * q + c] = Cx<R>{R(-0.5) * acc[q][c].re, R(-0.5) * acc[q][c].im};
-    eo_mass<true>(o, e);
-  } else {
-    Cx<R> bv[12];
-    b.load(h, bv);
-    if constexpr (EPI == EO_SCHUR) eo_mass<false>(bv, e);
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        o[3 * q + c] = Cx<R>{fma(R(0.5), acc[q][c].re, bv[3 * q + c].re), fma(R(0.5), acc[q][c].im, bv[3 * q + c].im)};
-    if constexpr (EPI == EO_SUB_MINV) eo_mass<true>(o, e);
   }
 }
 
@@ -367,6 +284,20 @@ static bool eo_two_rhs() {
   return on;
 }
 
+// FP16 batches of at least this many rhs run the M^-1 H and M b - H hops on
+// the matrix cores (k_eo_hop_mma); QCD_HIP_EO_MMA sets the threshold (0 =
+// never).  The default, 1, keeps every FP16 hop on one kernel, so a rhs gets
+// the same bits alone and in any batch.
+static int eo_mma_min_rhs() {
+  static const int n = [] {
+    const char *s = getenv("QCD_HIP_EO_MMA");
+    if (!s) return 1;
+    const int v = std::atoi(s);
+    return v <= 0 ? 1 << 30 : v;
+  }();
+  return n;
+}
+
 template <int PS, int PG>
 static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
                         int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st) {
@@ -375,6 +306,10 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
   const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
   const GaugeIn<PG> gu = GaugeIn<PG>::of(g);
 #define QH_E(E, RD) k_eo_hop<PS, PG, E, RD><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
+  if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16) {
+    if ((int)e.d.count >= eo_mma_min_rhs() && !red && (epi == EO_MINV || epi == EO_SCHUR))
+      return eo_mma_launch(epi, e, out, ro, in, ri, g, b, rb, st);
+  }
   if constexpr (PS != QCD_HIP_F64) {
     if (e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) && eo_two_rhs()) {
       const dim3 g2(e.d.nblk * ((e.d.count + 1) / 2));

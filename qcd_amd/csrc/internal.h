@@ -70,6 +70,8 @@ std::recursive_mutex &state_mutex();
 int ensure_init();
 int dev_alloc(void **p, size_t bytes);
 void dev_free(void *p, size_t bytes);
+// a library call wrote new links into gauge field f
+inline void gauge_written(qcd_hip_field f);
 
 }  // namespace qcdhip
 
@@ -92,9 +94,21 @@ struct qcd_hip_field_s {
   void *data;
   float *norm;       // F16 spinors only
   size_t bytes;      // total device bytes (data + norm)
+  // Gauge fields: every library write bumps version.  The matrix-core hops
+  // keep the phased links in operand order (mma.hip) built from version
+  // hop_version with the phases hop_ph; rebuilt when either changed.
+  unsigned version = 0;
+  void *hop_links = nullptr;
+  size_t hop_bytes = 0;
+  unsigned hop_version = 0;
+  double hop_ph[4][2] = {};
 };
 
 namespace qcdhip {
+
+inline void gauge_written(qcd_hip_field f) {
+  if (f) f->version++;
+}
 
 // --------------------------------------------------------- small helpers
 inline long round_up(long x, long m) { return (x + m - 1) / m * m; }

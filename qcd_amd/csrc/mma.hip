@@ -1,0 +1,405 @@
+// mma.hip -- multi-right-hand-side Wilson hops on the matrix cores (MFMA).
+//
+// For FP16-storage spinors (per-site norm) and FP16 links, batches of
+// right-hand sides apply each link to a panel of colour vectors with
+// v_mfma_f32_4x4x4_16b_f16: 16 independent 4x4x4 products per instruction,
+// one per site.  A wavefront owns 16 sites x 4 right-hand sides; lane
+// 4 b + n holds site b and rhs n.  For one hop of site b:
+//   A (4 x 4 per block, lane n holds row n): row i of the phased link
+//     W = e^{+-i theta/L} U (or U^+), as (re, -im) pairs for the real part of
+//     the product and (im, re) pairs for the imaginary part; row 3 is zero;
+//   B (4 x 4 per block, lane n holds column n): the spin-projected and
+//     rebuilt neighbour spinor chi_s = ((1 -+ g_mu) psi)_s of rhs n for one
+//     spin s, colours 0, 1 in the first k step, colour 2 (and a zero) in the
+//     second;
+//   D: lane n receives rows 0..2 (colours) of column n: Re or Im of
+//     (W chi_s)_c for its own site and rhs.
+// So one hop is 4 spins x {Re, Im} x 2 k steps = 16 MFMAs accumulating into
+// eight 4-float accumulators that hold the site's whole hopping sum: the
+// colour products, the rebuild of the lower spins and the sum over the eight
+// hops all run on the matrix cores.  The vector ALUs only project (six packed
+// FP16 adds per hop), rotate the two lower spins, scale by the site norms and
+// build W.  Each lane's output is its own (site, rhs), so the epilogues and
+// the FP16 store are those of the one-rhs kernels.
+//
+// Per-site norms: the MFMA inputs are FP16, so each neighbour spinor enters
+// scaled by n_nbr / n_ref, with n_ref the largest of the site's eight
+// neighbour norms for that rhs (all factors <= 1, no overflow), and the sum is
+// multiplied by n_ref at the end.
+//
+// The same hop core serves the full-lattice Dslash (D, Q) and the even-odd
+// hops (M^-1 H, M b - H) of the batched inner solve.
+#include "eo_common.h"
+
+namespace qcdhip {
+
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ h2v h2_zero() { return h2v{(_Float16)0.0f, (_Float16)0.0f}; }
+__device__ __forceinline__ h4v cat(h2v a, h2v b) { return h4v{a.x, a.y, b.x, b.y}; }
+
+// Per-lane view of one right-hand side of an FP16 spinor field: the lane's
+// rhs differs inside a wavefront, so base pointers are per lane.
+struct RhsF16 {
+  const char *base;   // component group 0, tile 0
+  const float *norm;
+  __device__ __forceinline__ void load(unsigned s, __half2 r[12], float &n) const {
+    const unsigned off = SpinorIn<QCD_HIP_F16>::offset(s);
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+      const u32x4 q = *(const u32x4 *)(base + g * 4 * kTile * 4 + off);
+#pragma unroll
+      for (int i = 0; i < 4; i++) r[4 * g + i] = as_half2(q[i]);
+    }
+    n = norm[s];
+  }
+  __device__ __forceinline__ float load_norm(unsigned s) const { return norm[s]; }
+};
+
+// ------------------------------------------------------------ hop links
+// The A operands depend only on the gauge field and the boundary phases, so
+// they are built once per field (and rebuilt when a library call rewrites the
+// field or the phases change): for direction mu and hop kind k (0: forward,
+// W = e^{+i theta_mu/L_mu} U_mu(x); 1: backward, W = e^{-i theta_mu/L_mu}
+// U_mu(x)^+, used by the site x + mu), row i of W as three (Re W_ic, -Im W_ic)
+// FP16 pairs, 12 bytes, at byte (site * 3 + i) * 12 of array 2 mu + k.  The Im
+// operand (Im W_ic, Re W_ic) is one packed multiply away.  288 bytes per site.
+struct HopLinks {
+  const char *p[8];   // array 2 mu + k
+};
+constexpr size_t kHopLinkBytes = 8 * 36;   // per site
+
+template <int PG>
+__global__ void __launch_bounds__(kBlock) k_hop_links(DslashArgs a, GaugeIn<PG> U, HopLinks out) {
+  const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (s >= a.V) return;
+  const int mu = blockIdx.y;
+  Cx<float> u[9];
+  {
+    typename Prec<PG>::T r[9];
+    U.load_raw(mu, s, r);
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = to_cx<float>(r[k]);
+  }
+  const float pr = (float)a.ph_re[mu], pi = (float)a.ph_im[mu];
+#pragma unroll
+  for (int kind = 0; kind < 2; kind++) {
+    // W = ph U (forward) or conj(ph) U^+ (backward)
+    const float qi = kind ? -pi : pi;
+    unsigned w[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const Cx<float> x = kind ? Cx<float>{u[3 * c + i].re, -u[3 * c + i].im} : u[3 * i + c];
+        const float wr = fmaf(x.re, pr, -(x.im * qi)), wi = fmaf(x.re, qi, x.im * pr);
+        w[3 * i + c] = as_u32(__floats2half2_rn(wr, -wi));
+      }
+    U32x3 *o = (U32x3 *)(const_cast<char *>(out.p[2 * mu + kind]) + (size_t)s * 36);
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[i] = U32x3{w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+  }
+}
+
+// The hop links of FP16 gauge field g with the phases of a, built on stream st
+// if g changed since they were last built or the phases differ.  Building
+// allocates, so it must not happen under graph capture: callers that capture
+// (the solver) prepare the links first.
+int hop_links_for(qcd_hip_field g, const DslashArgs &a, hipStream_t st, HopLinks *out) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());
+  bool same = g->hop_links && g->hop_version == g->version;
+  for (int mu = 0; mu < 4 && same; mu++) same = g->hop_ph[mu][0] == a.ph_re[mu] && g->hop_ph[mu][1] == a.ph_im[mu];
+  const size_t per = (size_t)36 * a.V;
+  if (!same) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    QH_CHECK(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      set_error("matrix-core hops: the hop links of this gauge field are not built (build them before capture)");
+      return 1;
+    }
+    const size_t bytes = 8 * per;
+    if (g->hop_bytes != bytes) {
+      dev_free(g->hop_links, g->hop_bytes);
+      g->hop_links = nullptr;
+      g->hop_bytes = 0;
+      QH_TRY(dev_alloc(&g->hop_links, bytes));
+      g->hop_bytes = bytes;
+    }
+    HopLinks hl;
+    for (int k = 0; k < 8; k++) hl.p[k] = (const char *)g->hop_links + k * per;
+    if (g->prec == QCD_HIP_F16)
+      k_hop_links<QCD_HIP_F16><<<dim3(grid_for(a.V), 4), kBlock, 0, st>>>(a, GaugeIn<QCD_HIP_F16>::of(g), hl);
+    else {
+      set_error("matrix-core hops: FP16 links expected");
+      return 1;
+    }
+    QH_CHECK(hipGetLastError());
+    g->hop_version = g->version;
+    for (int mu = 0; mu < 4; mu++) {
+      g->hop_ph[mu][0] = a.ph_re[mu];
+      g->hop_ph[mu][1] = a.ph_im[mu];
+    }
+  }
+  for (int k = 0; k < 8; k++) out->p[k] = (const char *)g->hop_links + k * per;
+  return 0;
+}
+
+// The two A operands (Re and Im parts) of one hop for link-row lane i: row
+// min(i, 2) of W (rows of A only reach the same rows of D, and row 3 of D is
+// never read, so the padding row may hold anything)
+__device__ __forceinline__ void link_operands(h4v are[2], h4v aim[2], const char *links, unsigned sl, unsigned i) {
+  const unsigned ii = i < 3 ? i : 2;
+  const U32x3 q = *(const U32x3 *)(links + ((size_t)sl * 3 + ii) * 12);
+  const unsigned w[3] = {q.x, q.y, q.z};
+  h2v re[3], im[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    re[c] = __builtin_bit_cast(h2v, w[c]);               // (Re W, -Im W)
+    im[c] = h2v{-re[c].y, re[c].x};                       // (Im W, Re W)
+  }
+  are[0] = cat(re[0], re[1]);
+  are[1] = cat(re[2], h2_zero());
+  aim[0] = cat(im[0], im[1]);
+  aim[1] = cat(im[2], h2_zero());
+}
+
+// D[s][0] += Re(W chi_s), D[s][1] += Im(W chi_s) for s = 0..3
+__device__ __forceinline__ void hop_mma(f4v D[4][2], const h4v are[2], const h4v aim[2], const h2v chi[4][3]) {
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const h4v b0 = cat(chi[s][0], chi[s][1]), b1 = cat(chi[s][2], h2_zero());
+    D[s][0] = __builtin_amdgcn_mfma_f32_4x4x4f16(are[0], b0, D[s][0], 0, 0, 0);
+    D[s][1] = __builtin_amdgcn_mfma_f32_4x4x4f16(aim[0], b0, D[s][1], 0, 0, 0);
+    D[s][0] = __builtin_amdgcn_mfma_f32_4x4x4f16(are[1], b1, D[s][0], 0, 0, 0);
+    D[s][1] = __builtin_amdgcn_mfma_f32_4x4x4f16(aim[1], b1, D[s][1], 0, 0, 0);
+  }
+}
+
+// chi_s = ((1 + S g_MU) psi)_s scaled by f, for the four spins (packed FP16)
+template <int MU, int S>
+__device__ __forceinline__ void project4(h2v chi[4][3], const __half2 r[12], h2v f) {
+  using PJ = Proj<MU>;
+  constexpr int sp = S > 0 ? 0 : 2;
+  h2v h0[3], h1[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h0[c] = (as_h2v(r[c]) + ipow_h<PJ::A + sp>(as_h2v(r[3 * PJ::P + c]))) * f;
+    h1[c] = (as_h2v(r[3 + c]) + ipow_h<PJ::B + sp>(as_h2v(r[3 * PJ::Q + c]))) * f;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    chi[0][c] = h0[c];
+    chi[1][c] = h1[c];
+    chi[2][c] = ipow_h<PJ::C + sp>(PJ::R == 0 ? h0[c] : h1[c]);
+    chi[3][c] = ipow_h<PJ::D + sp>(PJ::W == 0 ? h0[c] : h1[c]);
+  }
+}
+
+// Neighbour indices of one site: the spinor index of the forward and backward
+// neighbour in each direction and the site of the backward link.  EO: the
+// spinor index is the half-lattice entry (site >> 1).
+struct Hood {
+  unsigned nf[4], nb[4], lb[4];
+};
+template <bool EO>
+__device__ __forceinline__ Hood hood_of(const Site &st, const DslashArgs &a) {
+  Hood h;
+  const unsigned sh = EO ? 1 : 0;
+  h.nf[0] = neighbour<0, +1>(st, a) >> sh;
+  h.nf[1] = neighbour<1, +1>(st, a) >> sh;
+  h.nf[2] = neighbour<2, +1>(st, a) >> sh;
+  h.nf[3] = neighbour<3, +1>(st, a) >> sh;
+  h.lb[0] = neighbour<0, -1>(st, a);
+  h.lb[1] = neighbour<1, -1>(st, a);
+  h.lb[2] = neighbour<2, -1>(st, a);
+  h.lb[3] = neighbour<3, -1>(st, a);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) h.nb[mu] = h.lb[mu] >> sh;
+  return h;
+}
+
+// both hops of direction MU into D
+template <int MU>
+__device__ __forceinline__ void mma_dir(f4v D[4][2], const Site &st, const Hood &hd, const RhsF16 &in,
+                                        const HopLinks &hl, unsigned lane_n, float inv_ref) {
+  h4v are[2], aim[2];
+  h2v chi[4][3];
+  __half2 r[12];
+  float n = 0.f;
+  // forward: e^{+i th/L} U_mu(x) (1 - g_mu) psi(x + mu)
+  in.load(hd.nf[MU], r, n);
+  link_operands(are, aim, hl.p[2 * MU], st.s, lane_n);
+  {
+    const _Float16 f = (_Float16)(n * inv_ref);
+    project4<MU, -1>(chi, r, h2v{f, f});
+  }
+  hop_mma(D, are, aim, chi);
+  // backward: e^{-i th/L} U_mu(x - mu)^+ (1 + g_mu) psi(x - mu)
+  in.load(hd.nb[MU], r, n);
+  link_operands(are, aim, hl.p[2 * MU + 1], hd.lb[MU], lane_n);
+  {
+    const _Float16 f = (_Float16)(n * inv_ref);
+    project4<MU, +1>(chi, r, h2v{f, f});
+  }
+  hop_mma(D, are, aim, chi);
+}
+
+// The hopping sum H psi of this lane's (site, rhs), as acc[s][c] (already
+// multiplied back by the reference norm).
+template <bool EO>
+__device__ __forceinline__ void mma_hopping(Cx<float> acc[4][3], const DslashArgs &a, const Site &st,
+                                            const RhsF16 &in, const HopLinks &hl, unsigned lane_n) {
+  const Hood hd = hood_of<EO>(st, a);
+  float nref = 0.f;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) nref = fmaxf(nref, fmaxf(in.load_norm(hd.nf[mu]), in.load_norm(hd.nb[mu])));
+  // the scale factors cancel against nref below: an approximate reciprocal
+  // only moves each factor by an ulp of FP32
+  const float inv_ref = nref > 0.f ? __builtin_amdgcn_rcpf(nref) : 0.f;
+  f4v D[4][2];
+#pragma unroll
+  for (int s = 0; s < 4; s++) D[s][0] = D[s][1] = f4v{0.f, 0.f, 0.f, 0.f};
+  mma_dir<0>(D, st, hd, in, hl, lane_n, inv_ref);
+  mma_dir<1>(D, st, hd, in, hl, lane_n, inv_ref);
+  mma_dir<2>(D, st, hd, in, hl, lane_n, inv_ref);
+  mma_dir<3>(D, st, hd, in, hl, lane_n, inv_ref);
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[s][c] = Cx<float>{D[s][0][c] * nref, D[s][1][c] * nref};
+}
+
+// Wave -> (16-site block, group of 4 rhs): workgroups remapped per XCD, the
+// rhs group fastest, so the groups of one site block run together on one XCD
+// and read its links and neighbour planes through one L2.
+struct MmaTile {
+  unsigned blk, row;   // site block; batch row of this lane
+  bool valid;          // the wave has work
+  bool store;          // this lane's row exists (ragged last group)
+};
+__device__ __forceinline__ MmaTile mma_tile(unsigned nblk, unsigned count) {
+  const unsigned groups = (count + 3) / 4;
+  const unsigned w = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / 64) + threadIdx.x / 64;
+  const unsigned lane = threadIdx.x % 64;
+  MmaTile t;
+  t.valid = w < nblk * groups;
+  const unsigned g = w % groups;
+  t.blk = w / groups;
+  const unsigned row = 4 * g + (lane & 3);
+  t.store = row < count;
+  t.row = row < count ? row : count - 1;
+  return t;
+}
+
+// Full-lattice D or Q over a batch (FP16 spinors and links): lane 4 b + n
+// computes site 16 blk + b of batch row 4 g + n.
+template <int EPI>
+__global__ void __launch_bounds__(kBlock) k_dslash_mma(DslashArgs a, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
+                                                       SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
+  const MmaTile t = mma_tile(a.nblk, a.count);
+  if (!t.valid) return;   // whole wavefronts only
+  const unsigned lane = threadIdx.x % 64, n = lane & 3;
+  const unsigned s0 = 16 * t.blk + lane / 4;
+  const unsigned s = s0 < a.V ? s0 : a.V - 1;
+  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, t.row);
+  const RhsF16 in = RhsF16{(const char *)rv.in.p[0], rv.in.norm};
+  const Site st = site_coords(s, a);
+  Cx<float> acc[4][3];
+  mma_hopping<false>(acc, a, st, in, hl, n);
+  // out = (m + 4 + i tm g5) psi - 1/2 H psi
+  Cx<float> psi[12];
+  rv.in.load(s, psi);
+  const float mp4 = (float)a.mp4, tm = (float)a.tm;
+  Cx<float> o[12];
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++) {
+    const float g = sp < 2 ? tm : -tm;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const Cx<float> v = psi[3 * sp + c];
+      o[3 * sp + c] = Cx<float>{fmaf(-0.5f, acc[sp][c].re, fmaf(mp4, v.re, -(g * v.im))),
+                                fmaf(-0.5f, acc[sp][c].im, fmaf(mp4, v.im, g * v.re))};
+    }
+  }
+  apply_epilogue<EPI>(o, rv.b, s);
+  if (t.store && s0 < a.V) rv.out.template store<true>(s, o);
+}
+
+// Even-odd hop of an FP16 batch: output sites of parity e.par at
+// half-lattice entries 16 blk + b, neighbours read from the half field of the
+// other parity; epilogue M^-1 H in (EO_MINV) or M b - H in (EO_SCHUR).
+template <int EPI>
+__global__ void __launch_bounds__(kBlock) k_eo_hop_mma(EoArgs e, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
+                                                       SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
+  const DslashArgs &a = e.d;
+  const MmaTile t = mma_tile(a.nblk, a.count);
+  if (!t.valid) return;   // whole wavefronts only
+  const unsigned lane = threadIdx.x % 64, n = lane & 3;
+  const unsigned h0 = 16 * t.blk + lane / 4;
+  const unsigned h = h0 < e.Vh ? h0 : e.Vh - 1;
+  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, t.row);
+  const RhsF16 in = RhsF16{(const char *)rv.in.p[0], rv.in.norm};
+  const Site st = eo_site(h, e.par, a);
+  Cx<float> acc[4][3];
+  mma_hopping<true>(acc, a, st, in, hl, n);
+  Cx<float> o[12];
+  eo_epilogue<EPI>(o, acc, e, rv.b, h);
+  if (t.store && h0 < e.Vh) rv.out.template store<true>(h, o);
+}
+
+// build the hop links of FP16 gauge field g for the phases of lattice lat
+// (outside graph capture); no-op when they are current
+int mma_prepare(qcd_hip_field g, qcd_hip_lattice lat, hipStream_t st) {
+  if (!g || g->prec != QCD_HIP_F16) return 0;
+  const DslashArgs a = make_args(lat, 0.0, 0.0);
+  HopLinks hl;
+  return hop_links_for(g, a, st, &hl);
+}
+
+// workgroups of kBlock / 64 wavefronts over (site blocks of 16) x (rhs groups of 4)
+static unsigned mma_grid(unsigned nblk, unsigned count) {
+  const unsigned waves = nblk * ((count + 3) / 4), per = kBlock / 64;
+  return (waves + per - 1) / per;
+}
+
+int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
+                      qcd_hip_field b, int rb, hipStream_t st) {
+  if (in->prec != QCD_HIP_F16 || g->prec != QCD_HIP_F16 || (epi != EPI_D && epi != EPI_Q)) {
+    set_error("dslash: the matrix-core traversal needs FP16 spinors and links and the D or Q epilogue");
+    return 1;
+  }
+  a.nblk = (unsigned)((a.V + 15) / 16);
+  const unsigned grid = mma_grid(a.nblk, a.count);
+  const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
+  const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
+  const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);
+  HopLinks hl;
+  QH_TRY(hop_links_for(g, a, st, &hl));
+  if (epi == EPI_D) k_dslash_mma<EPI_D><<<grid, kBlock, 0, st>>>(a, si, hl, so, sb);
+  else k_dslash_mma<EPI_Q><<<grid, kBlock, 0, st>>>(a, si, hl, so, sb);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int eo_mma_launch(int epi, const EoArgs &e0, qcd_hip_field out, int ro, qcd_hip_field in, int ri, qcd_hip_field g,
+                  qcd_hip_field b, int rb, hipStream_t st) {
+  if (in->prec != QCD_HIP_F16 || g->prec != QCD_HIP_F16 || (epi != EO_MINV && epi != EO_SCHUR)) {
+    set_error("eo: the matrix-core hop needs FP16 spinors and links and the M^-1 H or M b - H epilogue");
+    return 1;
+  }
+  EoArgs e = e0;
+  e.d.nblk = (e.Vh + 15) / 16;
+  const unsigned grid = mma_grid(e.d.nblk, e.d.count);
+  const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
+  const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
+  const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);
+  HopLinks hl;
+  QH_TRY(hop_links_for(g, e.d, st, &hl));
+  if (epi == EO_MINV) k_eo_hop_mma<EO_MINV><<<grid, kBlock, 0, st>>>(e, si, hl, so, sb);
+  else k_eo_hop_mma<EO_SCHUR><<<grid, kBlock, 0, st>>>(e, si, hl, so, sb);
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace qcdhip

@@ -434,6 +434,7 @@ void qcd_hip_field_destroy(qcd_hip_field f) {
   if (!f) return;
   (void)hipStreamSynchronize(g_ctx.stream);
   dev_free(f->data, f->bytes);
+  dev_free(f->hop_links, f->hop_bytes);
   delete f;
 }
 
@@ -507,6 +508,7 @@ int qcd_hip_gauge_upload(qcd_hip_field f, const double *aos) {
     k_aos_to_planes<float2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (float2 *)f->data, V, st);
   else
     k_aos_to_planes<__half2, 36><<<grid_for(V), kBlock, 0, g_ctx.stream>>>(src, (__half2 *)f->data, V, st);
+  gauge_written(f);
   QH_CHECK(hipGetLastError());
   QH_CHECK(hipStreamSynchronize(g_ctx.stream));
   return 0;
@@ -567,6 +569,7 @@ int qcd_hip_field_convert(qcd_hip_field dst, qcd_hip_field src) {
   }
   const long V = src->lat->V, st = src->lat->stride;
   if (src->kind == QCD_HIP_GAUGE) {
+    gauge_written(dst);
     QH_TRY(convert_planes<36>(src->prec, src->data, nullptr, dst->prec, dst->data, nullptr, V, st));
   } else {
     for (int r = 0; r < src->nrhs; r++) {
@@ -636,6 +639,7 @@ int gauge_generate_device(qcd_hip_field g, double eps, uint64_t seed, hipStream_
     return 1;
   }
   const long V = g->lat->V;
+  gauge_written(g);
   k_gauge_generate<<<grid_for(4 * V), kBlock, 0, st>>>((double2 *)g->data, V, g->lat->stride, eps, seed);
   QH_CHECK(hipGetLastError());
   return 0;

@@ -23,7 +23,7 @@
 // Orthogonalisation is classical Gram-Schmidt in one fused pass (the
 // reference's modified Gram-Schmidt, invert.c:68-73, is the same projection in
 // exact arithmetic); the outer FP64 residual absorbs the difference.
-#include "blas_common.h"
+#include "eo_common.h"
 
 #include <chrono>
 #include <cmath>
@@ -31,10 +31,6 @@
 namespace qcdhip {
 
 // eo.hip
-enum { EO_MINV = 0, EO_SCHUR = 1, EO_SUB = 2, EO_SUB_MINV = 3 };
-int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
-              qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m, double tm,
-              unsigned par, double *partials, hipStream_t st, int *grid_out);
 int eo_split(qcd_hip_field full, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
              hipStream_t st);
 int eo_accumulate(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
@@ -641,6 +637,7 @@ size_t solver_device_bytes(const int L[4], const qcd_hip_solver_params *prm) {
   const size_t rowmax = ((size_t)2 * (n + kDotChunk) + 4) * (nb + 64);
   b += sizeof(double) * std::max(rowmax, (size_t)dslash_grid_bound(&lat)) * B;   // partials
   if (ip != QCD_HIP_F64) b += (size_t)36 * lat.stride * spinor_comp_bytes(ip);   // inner-precision links
+  if (ip == QCD_HIP_F16) b += (size_t)8 * 36 * lat.V;                            // matrix-core hop links
   return b;
 }
 }  // namespace qcdhip
@@ -755,6 +752,11 @@ int qcd_hip_solver_set_gauge(qcd_hip_solver S, qcd_hip_field g64) {
     QH_TRY(qcd_hip_gauge_create(&S->gin, S->lat, S->prm.inner_prec));
   }
   QH_TRY(qcd_hip_field_convert(S->gin, g64));
+  if (S->prm.inner_prec == QCD_HIP_F16) {
+    // the matrix-core hops read phased links prepared once per gauge field
+    QH_TRY(mma_prepare(S->gin, S->lat, ctx()->stream));
+    QH_CHECK(hipStreamSynchronize(ctx()->stream));
+  }
   return 0;
 }
 

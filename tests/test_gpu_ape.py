@@ -154,3 +154,46 @@ def test_dropin_stout_and_bad_arguments(gpu):
         Q.stout_smear(g, g, 0.1, 3, 1)
     with pytest.raises(Q.QcdError):
         Q.stout_smear(Q.Gauge(lat), g, 0.1, 5, 1)
+
+
+def _drand48_stream(seed, n):
+    """The libc drand48 sequence after srand48(seed): X <- (a X + c) mod 2^48."""
+    a, c, mask = 0x5DEECE66D, 0xB, (1 << 48) - 1
+    x = ((seed << 16) | 0x330E) & mask
+    out = np.empty(n)
+    for i in range(n):
+        x = (a * x + c) & mask
+        out[i] = x / float(1 << 48)
+    return out
+
+
+def test_project_su3_all_links_and_random_gauge_field(gpu):
+    """qcd_projectSU3 projects every link (all four directions) on the device,
+    link by link equal to the oracle's projection; qcd_randomGaugeField draws
+    drand48() entries in the reference's order (x, mu, c1, c2, re then im)
+    and projects them (lib/src/qcd_blas.c:1771-1782)."""
+    import ctypes
+    Q = gpu
+    L = (4, 2, 2, 2)
+    V = 32
+    rng = np.random.default_rng(91)
+    m = (rng.normal(size=(V, 4, 3, 3)) + 1j * rng.normal(size=(V, 4, 3, 3)))
+    di = Q.DropIn(L)
+    lib = Q.lib()
+    try:
+        u = di.gauge(m)
+        lib.qcd_projectSU3.argtypes = [ctypes.POINTER(Q.QcdField)]
+        lib.qcd_projectSU3(ctypes.byref(u))
+        got = di.view(u, 36).reshape(V, 4, 3, 3).copy()
+        assert np.abs(got - O.project_su3(m)).max() <= 1e-13
+        libc = ctypes.CDLL(None)
+        libc.srand48.argtypes = [ctypes.c_long]
+        libc.srand48(12345)
+        lib.qcd_randomGaugeField.argtypes = [ctypes.POINTER(Q.QcdField)]
+        lib.qcd_randomGaugeField(ctypes.byref(u))
+        d = _drand48_stream(12345, V * 72)
+        raw = (d[0::2] + 1j * d[1::2]).reshape(V, 4, 3, 3)
+        got = di.view(u, 36).reshape(V, 4, 3, 3).copy()
+        assert np.abs(got - O.project_su3(raw)).max() <= 1e-13
+    finally:
+        di.close()

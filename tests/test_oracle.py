@@ -486,3 +486,61 @@ def test_stout_smooths():
         u = O.stout(geo, u, 0.1, 4)
         p.append(O.plaquette(geo, u))
     assert p[0] < p[1] < p[2] < p[3] <= 1.0
+
+
+def _plaquette_numpy(L, u, shift_bug=False):
+    """Mean over sites and the 6 planes mu < nu of Re tr / 3 of
+    U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+, by array shifts (the lattice as
+    [z, y, x, t]).  shift_bug takes the third link at x+mu instead of x+nu."""
+    T, X, Y, Z = L
+    U = u.reshape(Z, Y, X, T, 4, 3, 3)
+    ax = {0: 3, 1: 2, 2: 1, 3: 0}
+
+    def fwd(a, mu):
+        return np.roll(a, -1, axis=ax[mu])
+
+    tot = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            a = U[..., mu, :, :]
+            b = fwd(U[..., nu, :, :], mu)
+            c = fwd(U[..., mu, :, :], mu if shift_bug else nu)
+            d = U[..., nu, :, :]
+            p = a @ b @ np.conj(np.swapaxes(c, -1, -2)) @ np.conj(np.swapaxes(d, -1, -2))
+            tot += np.trace(p, axis1=-2, axis2=-1).real.sum() / 3
+    return tot / (6 * T * X * Y * Z)
+
+
+def _random_su3(rng, n):
+    m = rng.normal(size=(n, 3, 3)) + 1j * rng.normal(size=(n, 3, 3))
+    q, r = np.linalg.qr(m)
+    q = q * (np.diagonal(r, axis1=1, axis2=2) / np.abs(np.diagonal(r, axis1=1, axis2=2)))[:, None, :]
+    return q / np.linalg.det(q)[:, None, None] ** (1 / 3)
+
+
+@pytest.mark.parametrize("L", [(4, 2, 6, 4), (2, 4, 4, 2)])
+def test_plaquette_brute_force_and_gauge_invariance(L):
+    """The oracle plaquette (lib/src/qcd_observables.c:15-36) equals a 4D
+    array-shift brute force over all six planes; a plaquette with a
+    mis-shifted link does not; and it is invariant under a random SU(3)
+    gauge transformation U_mu(x) -> g(x) U_mu(x) g(x+mu)^+."""
+    geo = O.Geometry(L)
+    u = O.gauge_generate(L, 0.7, 61)
+    p = O.plaquette(geo, u)
+    assert abs(p - _plaquette_numpy(L, u)) <= 1e-14
+    assert abs(p - _plaquette_numpy(L, u, shift_bug=True)) > 1e-3
+    T, X, Y, Z = L
+    V = geo.V
+    rng = np.random.default_rng(62)
+    g = _random_su3(rng, V)
+    assert np.abs(g @ np.conj(np.swapaxes(g, 1, 2)) - np.eye(3)).max() <= 1e-13
+    G = g.reshape(Z, Y, X, T, 3, 3)
+    U = u.reshape(Z, Y, X, T, 4, 3, 3)
+    ax = {0: 3, 1: 2, 2: 1, 3: 0}
+    Ug = np.empty_like(U)
+    for mu in range(4):
+        gs = np.roll(G, -1, axis=ax[mu])
+        Ug[..., mu, :, :] = G @ U[..., mu, :, :] @ np.conj(np.swapaxes(gs, -1, -2))
+    ug = np.ascontiguousarray(Ug.reshape(V, 4, 3, 3))
+    assert abs(O.plaquette(geo, ug) - p) <= 1e-14
+    assert O.plaquette(geo, np.tile(np.eye(3, dtype=complex), (V, 4, 1, 1))) == 1.0
