@@ -194,6 +194,19 @@ int  qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field gauge64, dou
 int  qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
                         double kappa, double mu, double *rel);
 
+/* The same residual without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The caller passes an FP64
+ * field r that receives b[rb] - D x[rx] in vector rr; the workgroup partial
+ * sums live in a static 2.0 MiB device array of the library (one per GPU,
+ * held until the process ends), so lattices whose Dslash launch has more than
+ * 2^18 workgroups are rejected with an error.  It uses the tuned launch
+ * configuration if there is one (no tuning here), synchronises the library
+ * stream to return *rel, and calls on one GPU that overlap race on the array.
+ * The library must be initialised (any field creation does that). */
+int  qcd_hip_b_minus_Dx_noalloc(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field gauge64,
+                                double kappa, double mu, qcd_hip_field r, int rr, double *rel);
+
 /* nsteps of 3d APE smearing (lib/src/qcd_smearing.c:205-273) of the FP64
  * gauge field in into out (distinct fields on one lattice): spatial links ->
  * P_SU3[U + alpha * spatial staples], temporal links copied.  nsteps = 0

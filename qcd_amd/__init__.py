@@ -142,6 +142,7 @@ def lib():
         "qcd_hip_solver_solve": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverStats)]),
         "qcd_hip_invert": (i, [_vp, _vp, _vp, d, d, ctypes.POINTER(SolverParams), ctypes.POINTER(SolverStats)]),
         "qcd_hip_b_minus_Dx": (i, [_vp, i, _vp, i, _vp, d, d, _dp]),
+        "qcd_hip_b_minus_Dx_noalloc": (i, [_vp, i, _vp, i, _vp, d, d, _vp, i, _dp]),
         "qcd_hip_gauge_generate": (i, [_vp, d, ctypes.c_uint64]),
         "qcd_hip_point_source": (i, [_vp, i, ctypes.POINTER(ctypes.c_int), i, i]),
         "qcd_hip_z4_source": (i, [_vp, i, i, ctypes.c_uint64]),
@@ -503,6 +504,14 @@ def b_minus_Dx(x, b, gauge64, kappa, mu=0.0, rx=0, rb=0):
     rel = np.zeros(1)
     _check(lib().qcd_hip_b_minus_Dx(x.h, int(rx), b.h, int(rb), gauge64.h, float(kappa), float(mu), _p(rel)),
            "qcd_hip_b_minus_Dx")
+    return float(rel[0])
+
+
+def b_minus_Dx_noalloc(x, b, gauge64, kappa, r, mu=0.0, rx=0, rb=0, rr=0):
+    """b_minus_Dx without device allocations; r[rr] receives b - D x."""
+    rel = np.zeros(1)
+    _check(lib().qcd_hip_b_minus_Dx_noalloc(x.h, int(rx), b.h, int(rb), gauge64.h, float(kappa), float(mu), r.h,
+                                            int(rr), _p(rel)), "qcd_hip_b_minus_Dx_noalloc")
     return float(rel[0])
 
 

@@ -203,6 +203,21 @@ __global__ void __launch_bounds__(kBlock) k_prop_scatter(SpinorSet<P> in, int r0
     for (int c1 = 0; c1 < 3; c1++) p[mu * 36 + c1 * 3] = make_double2((double)v[3 * mu + c1].re, (double)v[3 * mu + c1].im);
 }
 
+// |v[rhs]|^2 workgroup partials into partials[0 .. *nparts), asynchronous on st
+int norm2_launch(qcd_hip_field v, int rhs, double *partials, int *nparts, hipStream_t st) {
+  QH_TRY(check_rhs(v, rhs, "norm2"));
+  const unsigned V = (unsigned)v->lat->V;
+  const int g = reduce_grid(V);
+  switch (v->prec) {
+    case QCD_HIP_F64: k_norm2<QCD_HIP_F64><<<g, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F64>::of(v, rhs), V, partials); break;
+    case QCD_HIP_F32: k_norm2<QCD_HIP_F32><<<g, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F32>::of(v, rhs), V, partials); break;
+    default: k_norm2<QCD_HIP_F16><<<g, kBlock, 0, st>>>(SpinorIn<QCD_HIP_F16>::of(v, rhs), V, partials); break;
+  }
+  QH_CHECK(hipGetLastError());
+  *nparts = g;
+  return 0;
+}
+
 }  // namespace qcdhip
 
 using namespace qcdhip;

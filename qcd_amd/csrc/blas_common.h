@@ -89,6 +89,9 @@ int finalize(const double *partials, int nparts, int n_sums, double *out, hipStr
 // out[ro] = a x[rx] + b y[ry] (all sites, or slice t0 >= 0), asynchronous
 int lincomb_launch(qcd_hip_field out, int ro, double are, double aim, qcd_hip_field x, int rx, double bre, double bim,
                    qcd_hip_field y, int ry, int t0, hipStream_t st);
+// |v[rhs]|^2 workgroup partials (at most kMaxReduceBlocks) into partials,
+// their count in *nparts; asynchronous on st
+int norm2_launch(qcd_hip_field v, int rhs, double *partials, int *nparts, hipStream_t st);
 // cached scratch spinor number `slot` (resized on demand), owned by the library
 int scratch_spinor(int slot, qcd_hip_lattice lat, int prec, int nrhs, qcd_hip_field *out);
 // niter >= 1 iterations of Gauss smearing of every rhs of v on slice t0 (all

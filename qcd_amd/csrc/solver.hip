@@ -832,6 +832,85 @@ int qcd_hip_b_minus_Dx(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip
   return 0;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------ allocation-free residual
+// qcd_hip_b_minus_Dx_noalloc keeps its workgroup partial sums and the two
+// finished sums in this array instead of library allocations: at most
+// kNoAllocParts Dslash workgroups (every launch configuration of a 2^24-site
+// lattice fits), kMaxReduceBlocks norm partials and 2 results, 2.0 MiB per
+// GPU, held until the process ends.  In host code `scratch` only names the
+// symbol; kernels get its device address from noalloc_scratch().
+namespace {
+constexpr size_t kNoAllocParts = (size_t)1 << 18;
+constexpr size_t kNoAllocNorm = kNoAllocParts;                      // offset of the norm partials
+constexpr size_t kNoAllocOut = kNoAllocParts + kMaxReduceBlocks;    // offset of the two sums
+constexpr size_t kNoAllocDoubles = kNoAllocOut + 2;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices];
+
+// device address of `scratch` on the current device (looked up once per device)
+int noalloc_scratch(double **out) {
+  int dev = 0;
+  QH_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("b_minus_Dx_noalloc: device %d out of range", dev);
+    return 1;
+  }
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    QH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int qcd_hip_b_minus_Dx_noalloc(qcd_hip_field x, int rx, qcd_hip_field b, int rb, qcd_hip_field g, double kappa,
+                               double mu, qcd_hip_field r, int rr, double *rel) {
+  std::lock_guard<std::recursive_mutex> lock(state_mutex());   // the library stream
+  if (!ctx()->stream) {
+    set_error("b_minus_Dx_noalloc: the library is not initialised (create the fields first)");
+    return 1;
+  }
+  if (!x || !b || !g || !r || x->prec != QCD_HIP_F64 || b->prec != QCD_HIP_F64 || g->prec != QCD_HIP_F64 ||
+      r->prec != QCD_HIP_F64 || r->kind != QCD_HIP_SPINOR || !rel || rx < 0 || rx >= x->nrhs || rb < 0 ||
+      rb >= b->nrhs || rr < 0 || rr >= r->nrhs) {
+    set_error("b_minus_Dx_noalloc: needs FP64 x, b, r and gauge field and valid rhs indices");
+    return 1;
+  }
+  QH_TRY(dslash_check(r, x, g));
+  QH_TRY(dslash_check(x, b, g));
+  const int grid = dslash_grid(x, g);
+  if (grid < 1 || (size_t)grid > kNoAllocParts) {
+    set_error("b_minus_Dx_noalloc: %d Dslash workgroups exceed the %zu partial sums of the static scratch", grid,
+              kNoAllocParts);
+    return 1;
+  }
+  double *sp = nullptr;
+  QH_TRY(noalloc_scratch(&sp));
+  hipStream_t st = ctx()->stream;
+  int used = 0, nb = 0;
+  // r = b - D x with |r|^2 workgroup partials (every workgroup writes its slot)
+  QH_TRY(dslash_launch(EPI_RES_D, true, r, rr, x, rx, g, b, rb, 1.0 / (2.0 * kappa) - 4.0, mu, sp, st, &used));
+  if (used != grid) {
+    set_error("b_minus_Dx_noalloc: launch used %d workgroups, expected %d", used, grid);
+    return 1;
+  }
+  QH_TRY(norm2_launch(b, rb, sp + kNoAllocNorm, &nb, st));
+  QH_TRY(finalize(sp, used, 1, sp + kNoAllocOut, st));
+  QH_TRY(finalize(sp + kNoAllocNorm, nb, 1, sp + kNoAllocOut + 1, st));
+  double h[2] = {0.0, 0.0};
+  QH_CHECK(hipMemcpyAsync(h, sp + kNoAllocOut, sizeof(h), hipMemcpyDeviceToHost, st));
+  QH_CHECK(hipStreamSynchronize(st));
+  *rel = std::sqrt(h[0]) / std::sqrt(h[1]);
+  return 0;
+}
+
 // One-shot convenience: create a solver, solve every rhs of b, destroy it.
 int qcd_hip_invert(qcd_hip_field x, qcd_hip_field b, qcd_hip_field g64, double kappa, double mu,
                    const qcd_hip_solver_params *prm, qcd_hip_solver_stats *stats) {
