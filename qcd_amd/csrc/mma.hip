@@ -44,7 +44,8 @@ __device__ __forceinline__ h4v cat(h2v a, h2v b) { return h4v{a.x, a.y, b.x, b.y
 struct RhsF16 {
   const char *base;   // component group 0, tile 0
   const float *norm;
-  __device__ __forceinline__ void load(unsigned s, __half2 r[12], float &n) const {
+  // the stored values of site s (its norm is read separately)
+  __device__ __forceinline__ void load(unsigned s, __half2 r[12]) const {
     const unsigned off = SpinorIn<QCD_HIP_F16>::offset(s);
 #pragma unroll
     for (int g = 0; g < 3; g++) {
@@ -52,7 +53,6 @@ struct RhsF16 {
 #pragma unroll
       for (int i = 0; i < 4; i++) r[4 * g + i] = as_half2(q[i]);
     }
-    n = norm[s];
   }
   __device__ __forceinline__ float load_norm(unsigned s) const { return norm[s]; }
 };
@@ -60,11 +60,13 @@ struct RhsF16 {
 // ------------------------------------------------------------ hop links
 // The A operands depend only on the gauge field and the boundary phases, so
 // they are built once per field (and rebuilt when a library call rewrites the
-// field or the phases change): for direction mu and hop kind k (0: forward,
-// W = e^{+i theta_mu/L_mu} U_mu(x); 1: backward, W = e^{-i theta_mu/L_mu}
-// U_mu(x)^+, used by the site x + mu), row i of W as three (Re W_ic, -Im W_ic)
-// FP16 pairs, 12 bytes, at byte (site * 3 + i) * 12 of array 2 mu + k.  The Im
-// operand (Im W_ic, Re W_ic) is one packed multiply away.  288 bytes per site.
+// field or the phases change): for direction mu and hop kind k of site x
+// (0: forward, W = e^{+i theta_mu/L_mu} U_mu(x); 1: backward,
+// W = e^{-i theta_mu/L_mu} U_mu(x - mu)^+), row i of W as three
+// (Re W_ic, -Im W_ic) FP16 pairs, 12 bytes, at byte (x * 3 + i) * 12 of array
+// 2 mu + k.  Both kinds sit at the site that applies them, so a site's eight
+// link rows need only its own index.  The Im operand (Im W_ic, Re W_ic) is one
+// packed multiply away.  288 bytes per site.
 struct HopLinks {
   const char *p[8];   // array 2 mu + k
 };
@@ -75,17 +77,23 @@ __global__ void __launch_bounds__(kBlock) k_hop_links(DslashArgs a, GaugeIn<PG> 
   const unsigned s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (s >= a.V) return;
   const int mu = blockIdx.y;
-  Cx<float> u[9];
-  {
-    typename Prec<PG>::T r[9];
-    U.load_raw(mu, s, r);
-#pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = to_cx<float>(r[k]);
-  }
+  const Site st = site_coords(s, a);
+  const unsigned sb = mu == 0 ? neighbour<0, -1>(st, a)
+                      : mu == 1 ? neighbour<1, -1>(st, a)
+                      : mu == 2 ? neighbour<2, -1>(st, a)
+                                : neighbour<3, -1>(st, a);
   const float pr = (float)a.ph_re[mu], pi = (float)a.ph_im[mu];
 #pragma unroll
   for (int kind = 0; kind < 2; kind++) {
-    // W = ph U (forward) or conj(ph) U^+ (backward)
+    // W = ph U_mu(x) (forward) or conj(ph) U_mu(x - mu)^+ (backward), both
+    // stored at x, the site whose hop applies them
+    Cx<float> u[9];
+    {
+      typename Prec<PG>::T r[9];
+      U.load_raw(mu, kind ? sb : s, r);
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[k] = to_cx<float>(r[k]);
+    }
     const float qi = kind ? -pi : pi;
     unsigned w[9];
 #pragma unroll
@@ -145,12 +153,10 @@ int hop_links_for(qcd_hip_field g, const DslashArgs &a, hipStream_t st, HopLinks
   return 0;
 }
 
-// The two A operands (Re and Im parts) of one hop for link-row lane i: row
-// min(i, 2) of W (rows of A only reach the same rows of D, and row 3 of D is
-// never read, so the padding row may hold anything)
-__device__ __forceinline__ void link_operands(h4v are[2], h4v aim[2], const char *links, unsigned sl, unsigned i) {
-  const unsigned ii = i < 3 ? i : 2;
-  const U32x3 q = *(const U32x3 *)(links + ((size_t)sl * 3 + ii) * 12);
+// The two A operands (Re and Im parts) of one hop from the loaded link row q
+// (row min(i, 2) of W for link-row lane i: rows of A only reach the same rows
+// of D, and row 3 of D is never read, so the padding row may hold anything)
+__device__ __forceinline__ void link_operands(h4v are[2], h4v aim[2], const U32x3 &q) {
   const unsigned w[3] = {q.x, q.y, q.z};
   h2v re[3], im[3];
 #pragma unroll
@@ -176,17 +182,24 @@ __device__ __forceinline__ void hop_mma(f4v D[4][2], const h4v are[2], const h4v
   }
 }
 
-// chi_s = ((1 + S g_MU) psi)_s scaled by f, for the four spins (packed FP16)
+// Upper half spinor of (1 + S g_MU) psi scaled by f (packed FP16): h0, h1 are
+// spins 0 and 1; spins 2 and 3 are i^k rotations of them (rebuild4)
 template <int MU, int S>
-__device__ __forceinline__ void project4(h2v chi[4][3], const __half2 r[12], h2v f) {
+__device__ __forceinline__ void project2(h2v h0[3], h2v h1[3], const __half2 r[12], h2v f) {
   using PJ = Proj<MU>;
   constexpr int sp = S > 0 ? 0 : 2;
-  h2v h0[3], h1[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     h0[c] = (as_h2v(r[c]) + ipow_h<PJ::A + sp>(as_h2v(r[3 * PJ::P + c]))) * f;
     h1[c] = (as_h2v(r[3 + c]) + ipow_h<PJ::B + sp>(as_h2v(r[3 * PJ::Q + c]))) * f;
   }
+}
+
+// chi_s, the four spins of (1 + S g_MU) psi, from its upper half
+template <int MU, int S>
+__device__ __forceinline__ void rebuild4(h2v chi[4][3], const h2v h0[3], const h2v h1[3]) {
+  using PJ = Proj<MU>;
+  constexpr int sp = S > 0 ? 0 : 2;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     chi[0][c] = h0[c];
@@ -196,11 +209,30 @@ __device__ __forceinline__ void project4(h2v chi[4][3], const __half2 r[12], h2v
   }
 }
 
-// Neighbour indices of one site: the spinor index of the forward and backward
-// neighbour in each direction and the site of the backward link.  EO: the
-// spinor index is the half-lattice entry (site >> 1).
+// ------------------------------------------------------------ lane roles
+// A wavefront owns one block of 16 sites x one group of 4 batch rows, and
+// every lane plays two roles.  As a load lane, lane 16 n + b reads site b of
+// row n: 16 consecutive sites of one row, 256 contiguous bytes per component
+// group, so one load instruction covers four runs of whole cache lines.  The
+// load lane also reads the neighbour norms and projects and scales the
+// neighbour spinors of its (site, row).  As an MFMA lane, lane 4 b + n holds
+// site b, row n, the layout the matrix instruction fixes: it reads its site's
+// link rows and receives the six projected half2 of each hop from its load
+// lane through ds_bpermute.  Loading in the MFMA layout instead would put four
+// rows -- four separate fields -- in every four adjacent lanes: four L1 tag
+// lookups per lane group and far more address translations.
+__device__ __forceinline__ unsigned load_lane_of(unsigned lane) { return 16u * (lane & 3u) + (lane >> 2); }
+
+// value v of the lane whose byte address (4 x lane) is addr
+template <typename T>
+__device__ __forceinline__ T pull(T v, int addr) {
+  static_assert(sizeof(T) == 4, "one dword");
+  return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+
+// Neighbour spinor indices of one site (EO: half-lattice entries, site >> 1).
 struct Hood {
-  unsigned nf[4], nb[4], lb[4];
+  unsigned nf[4], nb[4];
 };
 template <bool EO>
 __device__ __forceinline__ Hood hood_of(const Site &st, const DslashArgs &a) {
@@ -210,103 +242,150 @@ __device__ __forceinline__ Hood hood_of(const Site &st, const DslashArgs &a) {
   h.nf[1] = neighbour<1, +1>(st, a) >> sh;
   h.nf[2] = neighbour<2, +1>(st, a) >> sh;
   h.nf[3] = neighbour<3, +1>(st, a) >> sh;
-  h.lb[0] = neighbour<0, -1>(st, a);
-  h.lb[1] = neighbour<1, -1>(st, a);
-  h.lb[2] = neighbour<2, -1>(st, a);
-  h.lb[3] = neighbour<3, -1>(st, a);
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) h.nb[mu] = h.lb[mu] >> sh;
+  h.nb[0] = neighbour<0, -1>(st, a) >> sh;
+  h.nb[1] = neighbour<1, -1>(st, a) >> sh;
+  h.nb[2] = neighbour<2, -1>(st, a) >> sh;
+  h.nb[3] = neighbour<3, -1>(st, a) >> sh;
   return h;
 }
 
-// both hops of direction MU into D
-template <int MU>
-__device__ __forceinline__ void mma_dir(f4v D[4][2], const Site &st, const Hood &hd, const RhsF16 &in,
-                                        const HopLinks &hl, unsigned lane_n, float inv_ref) {
-  h4v are[2], aim[2];
-  h2v chi[4][3];
+// What one hop reads: the neighbour's stored spinor (load lane) and the
+// lane's row of the hop link (MFMA lane).
+struct HopIn {
   __half2 r[12];
-  float n = 0.f;
-  // forward: e^{+i th/L} U_mu(x) (1 - g_mu) psi(x + mu)
-  in.load(hd.nf[MU], r, n);
-  link_operands(are, aim, hl.p[2 * MU], st.s, lane_n);
-  {
-    const _Float16 f = (_Float16)(n * inv_ref);
-    project4<MU, -1>(chi, r, h2v{f, f});
+  U32x3 w;
+};
+
+// issue the loads of hop (MU, K), K = 0 forward (neighbour x + mu), K = 1
+// backward (neighbour x - mu); sm: the MFMA lane's site, ii: its link row
+template <int MU, int K>
+__device__ __forceinline__ void hop_load(HopIn &h, const Hood &hd, const RhsF16 &in, const HopLinks &hl,
+                                         unsigned sm, unsigned ii) {
+  in.load(K ? hd.nb[MU] : hd.nf[MU], h.r);
+  h.w = *(const U32x3 *)(hl.p[2 * MU + K] + (sm * 36u + ii * 12u));   // below 2^32 for V <= 2^24
+}
+
+// D += W chi for hop (MU, K): forward e^{+i th/L} U_mu(x) (1 - g_mu) psi(x + mu),
+// backward e^{-i th/L} U_mu(x - mu)^+ (1 + g_mu) psi(x - mu); f: the load
+// lane's neighbour norm times 1 / n_ref; addr: byte address of the MFMA
+// lane's load lane
+template <int MU, int K>
+__device__ __forceinline__ void hop_apply(f4v D[4][2], const HopIn &h, float f, int addr) {
+  constexpr int S = K ? +1 : -1;
+  h2v h0[3], h1[3];
+  const _Float16 fh = (_Float16)f;
+  project2<MU, S>(h0, h1, h.r, h2v{fh, fh});
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h0[c] = pull(h0[c], addr);
+    h1[c] = pull(h1[c], addr);
   }
-  hop_mma(D, are, aim, chi);
-  // backward: e^{-i th/L} U_mu(x - mu)^+ (1 + g_mu) psi(x - mu)
-  in.load(hd.nb[MU], r, n);
-  link_operands(are, aim, hl.p[2 * MU + 1], hd.lb[MU], lane_n);
-  {
-    const _Float16 f = (_Float16)(n * inv_ref);
-    project4<MU, +1>(chi, r, h2v{f, f});
-  }
+  h4v are[2], aim[2];
+  link_operands(are, aim, h.w);
+  h2v chi[4][3];
+  rebuild4<MU, S>(chi, h0, h1);
   hop_mma(D, are, aim, chi);
 }
 
-// The hopping sum H psi of this lane's (site, rhs), as acc[s][c] (already
-// multiplied back by the reference norm).
+// The hopping sum H psi of the MFMA lane's (site, row), as acc[s][c] (already
+// multiplied back by the reference norm).  st: the load lane's site; in: the
+// load lane's row; sm: the MFMA lane's site (index of its hop links).
 template <bool EO>
 __device__ __forceinline__ void mma_hopping(Cx<float> acc[4][3], const DslashArgs &a, const Site &st,
-                                            const RhsF16 &in, const HopLinks &hl, unsigned lane_n) {
+                                            const RhsF16 &in, const HopLinks &hl, unsigned sm, unsigned lane) {
   const Hood hd = hood_of<EO>(st, a);
+  float nf[4], nb[4];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    nf[mu] = in.load_norm(hd.nf[mu]);
+    nb[mu] = in.load_norm(hd.nb[mu]);
+  }
+  // every load of the eight hops is issued before the first product
+  const unsigned ii = (lane & 3u) < 3u ? (lane & 3u) : 2u;
+  HopIn h[8];
+  hop_load<0, 0>(h[0], hd, in, hl, sm, ii);
+  hop_load<0, 1>(h[1], hd, in, hl, sm, ii);
+  hop_load<1, 0>(h[2], hd, in, hl, sm, ii);
+  hop_load<1, 1>(h[3], hd, in, hl, sm, ii);
+  hop_load<2, 0>(h[4], hd, in, hl, sm, ii);
+  hop_load<2, 1>(h[5], hd, in, hl, sm, ii);
+  hop_load<3, 0>(h[6], hd, in, hl, sm, ii);
+  hop_load<3, 1>(h[7], hd, in, hl, sm, ii);
+  __builtin_amdgcn_sched_barrier(0);   // keep the scheduler from sinking the loads
   float nref = 0.f;
 #pragma unroll
-  for (int mu = 0; mu < 4; mu++) nref = fmaxf(nref, fmaxf(in.load_norm(hd.nf[mu]), in.load_norm(hd.nb[mu])));
+  for (int mu = 0; mu < 4; mu++) nref = fmaxf(nref, fmaxf(nf[mu], nb[mu]));
   // the scale factors cancel against nref below: an approximate reciprocal
   // only moves each factor by an ulp of FP32
   const float inv_ref = nref > 0.f ? __builtin_amdgcn_rcpf(nref) : 0.f;
+  const int addr = (int)(4u * load_lane_of(lane));
   f4v D[4][2];
 #pragma unroll
   for (int s = 0; s < 4; s++) D[s][0] = D[s][1] = f4v{0.f, 0.f, 0.f, 0.f};
-  mma_dir<0>(D, st, hd, in, hl, lane_n, inv_ref);
-  mma_dir<1>(D, st, hd, in, hl, lane_n, inv_ref);
-  mma_dir<2>(D, st, hd, in, hl, lane_n, inv_ref);
-  mma_dir<3>(D, st, hd, in, hl, lane_n, inv_ref);
+  hop_apply<0, 0>(D, h[0], nf[0] * inv_ref, addr);
+  hop_apply<0, 1>(D, h[1], nb[0] * inv_ref, addr);
+  hop_apply<1, 0>(D, h[2], nf[1] * inv_ref, addr);
+  hop_apply<1, 1>(D, h[3], nb[1] * inv_ref, addr);
+  hop_apply<2, 0>(D, h[4], nf[2] * inv_ref, addr);
+  hop_apply<2, 1>(D, h[5], nb[2] * inv_ref, addr);
+  hop_apply<3, 0>(D, h[6], nf[3] * inv_ref, addr);
+  hop_apply<3, 1>(D, h[7], nb[3] * inv_ref, addr);
+  const float nm = pull(nref, addr);
 #pragma unroll
   for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) acc[s][c] = Cx<float>{D[s][0][c] * nref, D[s][1][c] * nref};
+    for (int c = 0; c < 3; c++) acc[s][c] = Cx<float>{D[s][0][c] * nm, D[s][1][c] * nm};
 }
 
-// Wave -> (16-site block, group of 4 rhs): workgroups remapped per XCD, the
-// rhs group fastest, so the groups of one site block run together on one XCD
-// and read its links and neighbour planes through one L2.
+// Wave -> (16-site block, group of 4 rhs): workgroups remapped per XCD.  The
+// rhs groups go in super-groups of sg: within one, the group is the fastest
+// index, so its groups of one site block run together on one XCD and read
+// the block's links through one L2; the site blocks of a super-group follow
+// each other.  sg = all groups shares the links most; a smaller sg keeps
+// fewer rhs' neighbour planes in L2 at a time.
 struct MmaTile {
-  unsigned blk, row;   // site block; batch row of this lane
-  bool valid;          // the wave has work
-  bool store;          // this lane's row exists (ragged last group)
+  unsigned blk, group;   // site block, group of 4 batch rows
+  bool valid;            // the wave has work
 };
-__device__ __forceinline__ MmaTile mma_tile(unsigned nblk, unsigned count) {
+__device__ __forceinline__ MmaTile mma_tile(unsigned nblk, unsigned count, unsigned sg) {
   const unsigned groups = (count + 3) / 4;
   const unsigned w = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / 64) + threadIdx.x / 64;
-  const unsigned lane = threadIdx.x % 64;
   MmaTile t;
   t.valid = w < nblk * groups;
-  const unsigned g = w % groups;
-  t.blk = w / groups;
-  const unsigned row = 4 * g + (lane & 3);
-  t.store = row < count;
-  t.row = row < count ? row : count - 1;
+  t.blk = t.group = 0;
+  if (!t.valid) return t;
+  const unsigned R = sg < 1 || sg > groups ? groups : sg;
+  const unsigned full = groups / R, span = nblk * R;
+  if (w < full * span) {
+    const unsigned q = w / span, r = w - q * span;
+    t.group = q * R + r % R;
+    t.blk = r / R;
+  } else {
+    const unsigned rem = groups - full * R, r = w - full * span;
+    t.group = full * R + r % rem;
+    t.blk = r / rem;
+  }
   return t;
 }
 
-// Full-lattice D or Q over a batch (FP16 spinors and links): lane 4 b + n
-// computes site 16 blk + b of batch row 4 g + n.
+// Full-lattice D or Q over a batch (FP16 spinors and links): the MFMA lane
+// 4 b + n computes site 16 blk + b of batch row 4 g + n.
 template <int EPI>
 __global__ void __launch_bounds__(kBlock) k_dslash_mma(DslashArgs a, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
                                                        SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
-  const MmaTile t = mma_tile(a.nblk, a.count);
+  const MmaTile t = mma_tile(a.nblk, a.count, a.rgroup);
   if (!t.valid) return;   // whole wavefronts only
-  const unsigned lane = threadIdx.x % 64, n = lane & 3;
-  const unsigned s0 = 16 * t.blk + lane / 4;
+  const unsigned lane = threadIdx.x % 64;
+  // load role: site lane % 16, row lane / 16 (rows past the batch read the last row)
+  const unsigned ll = 16 * t.blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
+  const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
+  const Site st = site_coords(ll < a.V ? ll : a.V - 1, a);
+  // MFMA role: site lane / 4, row lane % 4
+  const unsigned s0 = 16 * t.blk + (lane >> 2), rm = 4 * t.group + (lane & 3u);
   const unsigned s = s0 < a.V ? s0 : a.V - 1;
-  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, t.row);
-  const RhsF16 in = RhsF16{(const char *)rv.in.p[0], rv.in.norm};
-  const Site st = site_coords(s, a);
   Cx<float> acc[4][3];
-  mma_hopping<false>(acc, a, st, in, hl, n);
+  mma_hopping<false>(acc, a, st, RhsF16{(const char *)lv.in.p[0], lv.in.norm}, hl, s, lane);
+  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, rm < a.count ? rm : a.count - 1);
   // out = (m + 4 + i tm g5) psi - 1/2 H psi
   Cx<float> psi[12];
   rv.in.load(s, psi);
@@ -323,7 +402,7 @@ __global__ void __launch_bounds__(kBlock) k_dslash_mma(DslashArgs a, SpinorIn<QC
     }
   }
   apply_epilogue<EPI>(o, rv.b, s);
-  if (t.store && s0 < a.V) rv.out.template store<true>(s, o);
+  if (rm < a.count && s0 < a.V) rv.out.template store<true>(s, o);
 }
 
 // Even-odd hop of an FP16 batch: output sites of parity e.par at
@@ -333,19 +412,22 @@ template <int EPI>
 __global__ void __launch_bounds__(kBlock) k_eo_hop_mma(EoArgs e, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
                                                        SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
   const DslashArgs &a = e.d;
-  const MmaTile t = mma_tile(a.nblk, a.count);
+  const MmaTile t = mma_tile(a.nblk, a.count, a.rgroup);
   if (!t.valid) return;   // whole wavefronts only
-  const unsigned lane = threadIdx.x % 64, n = lane & 3;
-  const unsigned h0 = 16 * t.blk + lane / 4;
+  const unsigned lane = threadIdx.x % 64;
+  const unsigned hl0 = 16 * t.blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
+  const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
+  const Site st = eo_site(hl0 < e.Vh ? hl0 : e.Vh - 1, e.par, a);
+  const unsigned h0 = 16 * t.blk + (lane >> 2), rm = 4 * t.group + (lane & 3u);
   const unsigned h = h0 < e.Vh ? h0 : e.Vh - 1;
-  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, t.row);
-  const RhsF16 in = RhsF16{(const char *)rv.in.p[0], rv.in.norm};
-  const Site st = eo_site(h, e.par, a);
+  // the MFMA lane's full-lattice site: that of load lane b (row 0 of the group)
+  const unsigned sm = pull(st.s, (int)(4u * (lane >> 2)));
   Cx<float> acc[4][3];
-  mma_hopping<true>(acc, a, st, in, hl, n);
+  mma_hopping<true>(acc, a, st, RhsF16{(const char *)lv.in.p[0], lv.in.norm}, hl, sm, lane);
+  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, rm < a.count ? rm : a.count - 1);
   Cx<float> o[12];
   eo_epilogue<EPI>(o, acc, e, rv.b, h);
-  if (t.store && h0 < e.Vh) rv.out.template store<true>(h, o);
+  if (rm < a.count && h0 < e.Vh) rv.out.template store<true>(h, o);
 }
 
 // build the hop links of FP16 gauge field g for the phases of lattice lat
@@ -355,6 +437,16 @@ int mma_prepare(qcd_hip_field g, qcd_hip_lattice lat, hipStream_t st) {
   const DslashArgs a = make_args(lat, 0.0, 0.0);
   HopLinks hl;
   return hop_links_for(g, a, st, &hl);
+}
+
+// rhs groups per super-group (mma_tile); QCD_HIP_MMA_RGROUP overrides the
+// default, all groups
+static unsigned mma_rgroup() {
+  static const unsigned r = [] {
+    const char *e = getenv("QCD_HIP_MMA_RGROUP");
+    return e ? (unsigned)std::atoi(e) : 0u;
+  }();
+  return r;
 }
 
 // workgroups of kBlock / 64 wavefronts over (site blocks of 16) x (rhs groups of 4)
@@ -370,6 +462,7 @@ int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_
     return 1;
   }
   a.nblk = (unsigned)((a.V + 15) / 16);
+  a.rgroup = mma_rgroup();
   const unsigned grid = mma_grid(a.nblk, a.count);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
@@ -390,6 +483,7 @@ int eo_mma_launch(int epi, const EoArgs &e0, qcd_hip_field out, int ro, qcd_hip_
   }
   EoArgs e = e0;
   e.d.nblk = (e.Vh + 15) / 16;
+  e.d.rgroup = mma_rgroup();
   const unsigned grid = mma_grid(e.d.nblk, e.d.count);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;

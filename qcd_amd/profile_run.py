@@ -29,6 +29,8 @@ def main(argv=None):
     ap.add_argument("--lattice", type=int, nargs=4, default=[48, 24, 24, 24], help="T X Y Z")
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16"])
     ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed launches first (0 for counter runs, so every dispatch is a timed one)")
     ap.add_argument("--op", default="D", choices=["D", "Q"])
     ap.add_argument("--map", type=int, default=None,
                     help="force a traversal (0 XCD ranges, 1 y-slabs, 2 + 256*n z-march of n steps)")
@@ -67,7 +69,7 @@ def main(argv=None):
         print("tuned:", Q.dslash_tune(y, x, g), flush=True)
     if args.map is not None:
         Q.dslash_set_config(x, g, args.map, args.occ)
-    ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_Q if args.op == "Q" else Q.OP_D, warmup=0, iters=args.iters)
+    ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_Q if args.op == "Q" else Q.OP_D, warmup=args.warmup, iters=args.iters)
     a = Q.Spinor(lat).upload(psi)
     b = Q.Spinor(lat)
     for _ in range(args.iters):
