@@ -31,6 +31,8 @@
 // hops (M^-1 H, M b - H) of the batched inner solve.
 #include "eo_common.h"
 
+#include <numeric>
+
 namespace qcdhip {
 
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
@@ -230,204 +232,246 @@ __device__ __forceinline__ T pull(T v, int addr) {
   return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
 }
 
-// Neighbour spinor indices of one site (EO: half-lattice entries, site >> 1).
-struct Hood {
-  unsigned nf[4], nb[4];
-};
-template <bool EO>
-__device__ __forceinline__ Hood hood_of(const Site &st, const DslashArgs &a) {
-  Hood h;
-  const unsigned sh = EO ? 1 : 0;
-  h.nf[0] = neighbour<0, +1>(st, a) >> sh;
-  h.nf[1] = neighbour<1, +1>(st, a) >> sh;
-  h.nf[2] = neighbour<2, +1>(st, a) >> sh;
-  h.nf[3] = neighbour<3, +1>(st, a) >> sh;
-  h.nb[0] = neighbour<0, -1>(st, a) >> sh;
-  h.nb[1] = neighbour<1, -1>(st, a) >> sh;
-  h.nb[2] = neighbour<2, -1>(st, a) >> sh;
-  h.nb[3] = neighbour<3, -1>(st, a) >> sh;
-  return h;
-}
-
-// What one hop reads: the neighbour's stored spinor (load lane) and the
-// lane's row of the hop link (MFMA lane).
-struct HopIn {
-  __half2 r[12];
-  U32x3 w;
-};
-
-// issue the loads of hop (MU, K), K = 0 forward (neighbour x + mu), K = 1
-// backward (neighbour x - mu); sm: the MFMA lane's site, ii: its link row
-template <int MU, int K>
-__device__ __forceinline__ void hop_load(HopIn &h, const Hood &hd, const RhsF16 &in, const HopLinks &hl,
-                                         unsigned sm, unsigned ii) {
-  in.load(K ? hd.nb[MU] : hd.nf[MU], h.r);
-  h.w = *(const U32x3 *)(hl.p[2 * MU + K] + (sm * 36u + ii * 12u));   // below 2^32 for V <= 2^24
-}
-
 // D += W chi for hop (MU, K): forward e^{+i th/L} U_mu(x) (1 - g_mu) psi(x + mu),
-// backward e^{-i th/L} U_mu(x - mu)^+ (1 + g_mu) psi(x - mu); f: the load
-// lane's neighbour norm times 1 / n_ref; addr: byte address of the MFMA
-// lane's load lane
+// backward e^{-i th/L} U_mu(x - mu)^+ (1 + g_mu) psi(x - mu).  r: the load
+// lane's neighbour spinor, f: its norm times 1 / n_ref; w: the MFMA lane's
+// link row; addr: byte address of the MFMA lane's load lane
 template <int MU, int K>
-__device__ __forceinline__ void hop_apply(f4v D[4][2], const HopIn &h, float f, int addr) {
+__device__ __forceinline__ void hop_apply(f4v D[4][2], const __half2 r[12], const U32x3 &w, float f, int addr) {
   constexpr int S = K ? +1 : -1;
   h2v h0[3], h1[3];
   const _Float16 fh = (_Float16)f;
-  project2<MU, S>(h0, h1, h.r, h2v{fh, fh});
+  project2<MU, S>(h0, h1, r, h2v{fh, fh});
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     h0[c] = pull(h0[c], addr);
     h1[c] = pull(h1[c], addr);
   }
   h4v are[2], aim[2];
-  link_operands(are, aim, h.w);
+  link_operands(are, aim, w);
   h2v chi[4][3];
   rebuild4<MU, S>(chi, h0, h1);
   hop_mma(D, are, aim, chi);
 }
 
-// The hopping sum H psi of the MFMA lane's (site, row), as acc[s][c] (already
-// multiplied back by the reference norm).  st: the load lane's site; in: the
-// load lane's row; sm: the MFMA lane's site (index of its hop links).
-template <bool EO>
-__device__ __forceinline__ void mma_hopping(Cx<float> acc[4][3], const DslashArgs &a, const Site &st,
-                                            const RhsF16 &in, const HopLinks &hl, unsigned sm, unsigned lane) {
-  const Hood hd = hood_of<EO>(st, a);
-  float nf[4], nb[4];
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) {
-    nf[mu] = in.load_norm(hd.nf[mu]);
-    nb[mu] = in.load_norm(hd.nb[mu]);
-  }
-  // every load of the eight hops is issued before the first product
-  const unsigned ii = (lane & 3u) < 3u ? (lane & 3u) : 2u;
-  HopIn h[8];
-  hop_load<0, 0>(h[0], hd, in, hl, sm, ii);
-  hop_load<0, 1>(h[1], hd, in, hl, sm, ii);
-  hop_load<1, 0>(h[2], hd, in, hl, sm, ii);
-  hop_load<1, 1>(h[3], hd, in, hl, sm, ii);
-  hop_load<2, 0>(h[4], hd, in, hl, sm, ii);
-  hop_load<2, 1>(h[5], hd, in, hl, sm, ii);
-  hop_load<3, 0>(h[6], hd, in, hl, sm, ii);
-  hop_load<3, 1>(h[7], hd, in, hl, sm, ii);
-  __builtin_amdgcn_sched_barrier(0);   // keep the scheduler from sinking the loads
-  float nref = 0.f;
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) nref = fmaxf(nref, fmaxf(nf[mu], nb[mu]));
-  // the scale factors cancel against nref below: an approximate reciprocal
-  // only moves each factor by an ulp of FP32
-  const float inv_ref = nref > 0.f ? __builtin_amdgcn_rcpf(nref) : 0.f;
-  const int addr = (int)(4u * load_lane_of(lane));
-  f4v D[4][2];
-#pragma unroll
-  for (int s = 0; s < 4; s++) D[s][0] = D[s][1] = f4v{0.f, 0.f, 0.f, 0.f};
-  hop_apply<0, 0>(D, h[0], nf[0] * inv_ref, addr);
-  hop_apply<0, 1>(D, h[1], nb[0] * inv_ref, addr);
-  hop_apply<1, 0>(D, h[2], nf[1] * inv_ref, addr);
-  hop_apply<1, 1>(D, h[3], nb[1] * inv_ref, addr);
-  hop_apply<2, 0>(D, h[4], nf[2] * inv_ref, addr);
-  hop_apply<2, 1>(D, h[5], nb[2] * inv_ref, addr);
-  hop_apply<3, 0>(D, h[6], nf[3] * inv_ref, addr);
-  hop_apply<3, 1>(D, h[7], nb[3] * inv_ref, addr);
-  const float nm = pull(nref, addr);
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) acc[s][c] = Cx<float>{D[s][0][c] * nm, D[s][1][c] * nm};
-}
-
-// Wave -> (16-site block, group of 4 rhs): workgroups remapped per XCD.  The
-// rhs groups go in super-groups of sg: within one, the group is the fastest
-// index, so its groups of one site block run together on one XCD and read
-// the block's links through one L2; the site blocks of a super-group follow
-// each other.  sg = all groups shares the links most; a smaller sg keeps
-// fewer rhs' neighbour planes in L2 at a time.
-struct MmaTile {
-  unsigned blk, group;   // site block, group of 4 batch rows
-  bool valid;            // the wave has work
+// ------------------------------------------------------------ z-march
+// A wavefront owns 16 consecutive (t, x, y) columns -- even-odd: half
+// columns, the pairs (2 j, 2 j + 1) of t with x, y fixed -- of one group of 4
+// batch rows and steps through zlen consecutive planes z.  Its load lanes keep
+// the stored input spinors of their column at planes z - 1 and z (loaded as
+// forward z neighbours one and two steps earlier) and load plane z + 1, so
+// every input plane of a column is read once per z chunk plus two planes of
+// halo.  Plane z - 1 is the backward z neighbour and plane z the centre of the
+// full-lattice operator.  Even-odd: the output site of half column j at plane
+// z has its z neighbours at half column j of the input field, planes z -+ 1,
+// so the same ring serves the hops.  The epilogue runs in the load lanes
+// (the hopping sum comes back from the MFMA layout by ds_bpermute), so the
+// output stores are as coalesced as the loads.
+struct Plane {
+  __half2 r[12];
+  float n;
 };
-__device__ __forceinline__ MmaTile mma_tile(unsigned nblk, unsigned count, unsigned sg) {
-  const unsigned groups = (count + 3) / 4;
+
+// Wave -> (group of 4 rhs, column block, z chunk): workgroups remapped per
+// XCD.  The rhs groups go in super-groups of sg: within one, the group is the
+// fastest index, so the groups of one column block run together on one XCD
+// and read the block's links through one L2; then the column blocks of a z
+// chunk follow each other, then the z chunks.  sg = all groups shares the
+// links most; a smaller sg keeps more columns of fewer rhs in flight, so the
+// y neighbours of a plane are read while still in L2.
+struct MmaTile {
+  unsigned blk, zc, group;   // column block, z chunk, group of 4 batch rows
+  bool valid;                // the wave has work
+};
+__device__ __forceinline__ MmaTile mma_tile(unsigned ncb, unsigned nzc, unsigned count, unsigned sg) {
+  const unsigned groups = (count + 3) / 4, nblk = ncb * nzc;
   const unsigned w = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / 64) + threadIdx.x / 64;
   MmaTile t;
   t.valid = w < nblk * groups;
-  t.blk = t.group = 0;
+  t.blk = t.zc = t.group = 0;
   if (!t.valid) return t;
   const unsigned R = sg < 1 || sg > groups ? groups : sg;
   const unsigned full = groups / R, span = nblk * R;
+  unsigned rest;
   if (w < full * span) {
     const unsigned q = w / span, r = w - q * span;
     t.group = q * R + r % R;
-    t.blk = r / R;
+    rest = r / R;
   } else {
     const unsigned rem = groups - full * R, r = w - full * span;
     t.group = full * R + r % rem;
-    t.blk = r / rem;
+    rest = r / rem;
   }
+  t.zc = rest / ncb;
+  t.blk = rest - t.zc * ncb;
   return t;
 }
 
-// Full-lattice D or Q over a batch (FP16 spinors and links): the MFMA lane
-// 4 b + n computes site 16 blk + b of batch row 4 g + n.
+// Physical column block of the o-th block of a plane in traversal order.  In
+// index order the y neighbours of a block lie L0 L1 / 16 blocks away, and the
+// wave that reads them as its own columns runs that many places earlier or
+// later: many z steps apart in a z-march, long enough for the plane to leave
+// L2.  The tiled order runs x-tiles of wu units (a unit: the whole x-rows
+// that bpu blocks fill exactly) with y outside the units of a tile, so a
+// block's y neighbours are bpu wu blocks away; x neighbours across a tile
+// edge are the price.
+__device__ __forceinline__ unsigned tile_block(unsigned o, const DslashArgs &a) {
+  if (a.tile_bpu == 0) return o;
+  const unsigned bpu = a.tile_bpu, wu = a.tile_wu;
+  const unsigned j = o % bpu, r = o / bpu;
+  const unsigned ui = r % wu, r2 = r / wu;
+  const unsigned y = r2 % a.L[2], xt = r2 / a.L[2];
+  return j + bpu * (xt * wu + ui + a.tile_upr * y);
+}
+
+// The z-march body shared by the full-lattice and even-odd kernels.  PP:
+// columns per plane (EO: half columns); par: parity of the output sites (EO).
+// epi(acc, centre, s, h, view, store) finishes one site in the load lane:
+// acc = H psi of its (site, row), centre = input plane z of its column, s =
+// full-lattice output site, h = its half-lattice entry (EO), view = the row's
+// fields, store = the site and row exist.
+template <bool EO, typename Epi>
+__device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, const SpinorIn<QCD_HIP_F16> &in_all,
+                                          const HopLinks &hl, const SpinorOut<QCD_HIP_F16> &out_all,
+                                          const SpinorIn<QCD_HIP_F16> &b_all, Epi epi) {
+  const unsigned P3 = a.L[0] * a.L[1] * a.L[2], Lz = a.L[3];
+  const unsigned PP = EO ? P3 / 2 : P3;
+  const MmaTile t = mma_tile(a.nblk, (Lz + a.zlen - 1) / a.zlen, a.count, a.rgroup);
+  if (!t.valid) return;   // whole wavefronts only
+  const unsigned lane = threadIdx.x % 64;
+  // load role: column lane % 16 of the block, row lane / 16 of the group
+  // (columns and rows past the end read the last ones and store nothing)
+  const unsigned blk = tile_block(t.blk, a);
+  const unsigned cl0 = 16 * blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
+  const bool store = cl0 < PP && rl < a.count;
+  const unsigned cl = cl0 < PP ? cl0 : PP - 1;
+  const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
+  const RhsF16 in{(const char *)lv.in.p[0], lv.in.norm};
+  Site st;
+  {
+    const unsigned l0 = EO ? 2 * cl : cl;
+    const unsigned r = l0 / a.L[0];
+    st.t = l0 - r * a.L[0];
+    st.y = r / a.L[1];
+    st.x = r - st.y * a.L[1];
+  }
+  const unsigned t0 = st.t;
+  // MFMA role: column lane / 4, row lane % 4
+  const unsigned cm0 = 16 * blk + (lane >> 2);
+  const unsigned cm = cm0 < PP ? cm0 : PP - 1;
+  const int addr = (int)(4u * load_lane_of(lane));                      // MFMA lane <- load lane
+  const int back = (int)(4u * (4u * (lane & 15u) + (lane >> 4)));     // load lane <- MFMA lane
+  const int bsite = (int)(4u * (lane >> 2));                            // load lane of row 0, same column
+  const unsigned ii = (lane & 3u) < 3u ? (lane & 3u) : 2u;
+  const unsigned z0 = t.zc * a.zlen, z1 = min(z0 + a.zlen, Lz);
+  Plane pm, pc;   // input planes z - 1 and z of the load lane's column
+  {
+    const unsigned zm = z0 == 0 ? Lz - 1 : z0 - 1;
+    in.load(cl + PP * zm, pm.r);
+    pm.n = in.load_norm(cl + PP * zm);
+    in.load(cl + PP * z0, pc.r);
+    pc.n = in.load_norm(cl + PP * z0);
+  }
+  const unsigned sh = EO ? 1 : 0;
+  for (unsigned z = z0; z < z1; z++) {
+    st.z = z;
+    if constexpr (EO) {
+      const unsigned odd = (par + st.x + st.y + z) & 1u;
+      st.t = t0 + odd;
+      st.s = 2 * cl + P3 * z + odd;
+    } else {
+      st.s = cl + P3 * z;
+    }
+    const unsigned sm = EO ? pull(st.s, bsite) : cm + P3 * z;
+    const unsigned ip = cl + PP * (z + 1 == Lz ? 0 : z + 1);
+    const unsigned nbr[6] = {neighbour<0, +1>(st, a) >> sh, neighbour<0, -1>(st, a) >> sh,
+                             neighbour<1, +1>(st, a) >> sh, neighbour<1, -1>(st, a) >> sh,
+                             neighbour<2, +1>(st, a) >> sh, neighbour<2, -1>(st, a) >> sh};
+    // every load of the step is issued before the first product (splitting
+    // the step into two halves whose loads fly one half ahead needs about 300
+    // registers: one wavefront per SIMD, measured slower)
+    float n[7];
+#pragma unroll
+    for (int k = 0; k < 6; k++) n[k] = in.load_norm(nbr[k]);
+    n[6] = in.load_norm(ip);
+    __half2 r[7][12];
+    U32x3 w[8];
+#pragma unroll
+    for (int k = 0; k < 6; k++) in.load(nbr[k], r[k]);
+    in.load(ip, r[6]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = *(const U32x3 *)(hl.p[k] + (sm * 36u + ii * 12u));   // below 2^32 for V <= 2^24
+    __builtin_amdgcn_sched_barrier(0);   // keep the scheduler from sinking the loads
+    float nref = pm.n;
+#pragma unroll
+    for (int k = 0; k < 7; k++) nref = fmaxf(nref, n[k]);
+    // the scale factors cancel against nref below: an approximate reciprocal
+    // only moves each factor by an ulp of FP32
+    const float inv_ref = nref > 0.f ? __builtin_amdgcn_rcpf(nref) : 0.f;
+    f4v D[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) D[q][0] = D[q][1] = f4v{0.f, 0.f, 0.f, 0.f};
+    hop_apply<0, 0>(D, r[0], w[0], n[0] * inv_ref, addr);
+    hop_apply<0, 1>(D, r[1], w[1], n[1] * inv_ref, addr);
+    hop_apply<1, 0>(D, r[2], w[2], n[2] * inv_ref, addr);
+    hop_apply<1, 1>(D, r[3], w[3], n[3] * inv_ref, addr);
+    hop_apply<2, 0>(D, r[4], w[4], n[4] * inv_ref, addr);
+    hop_apply<2, 1>(D, r[5], w[5], n[5] * inv_ref, addr);
+    hop_apply<3, 0>(D, r[6], w[6], n[6] * inv_ref, addr);
+    hop_apply<3, 1>(D, pm.r, w[7], pm.n * inv_ref, addr);
+    Cx<float> acc[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[q][c] = Cx<float>{pull(D[q][0][c], back) * nref, pull(D[q][1][c], back) * nref};
+    epi(acc, pc, st.s, cl + PP * z, lv, store);
+    pm = pc;
+#pragma unroll
+    for (int k = 0; k < 12; k++) pc.r[k] = r[6][k];
+    pc.n = n[6];
+  }
+}
+
+// Full-lattice D or Q over a batch (FP16 spinors and links).
 template <int EPI>
 __global__ void __launch_bounds__(kBlock) k_dslash_mma(DslashArgs a, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
                                                        SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
-  const MmaTile t = mma_tile(a.nblk, a.count, a.rgroup);
-  if (!t.valid) return;   // whole wavefronts only
-  const unsigned lane = threadIdx.x % 64;
-  // load role: site lane % 16, row lane / 16 (rows past the batch read the last row)
-  const unsigned ll = 16 * t.blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
-  const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
-  const Site st = site_coords(ll < a.V ? ll : a.V - 1, a);
-  // MFMA role: site lane / 4, row lane % 4
-  const unsigned s0 = 16 * t.blk + (lane >> 2), rm = 4 * t.group + (lane & 3u);
-  const unsigned s = s0 < a.V ? s0 : a.V - 1;
-  Cx<float> acc[4][3];
-  mma_hopping<false>(acc, a, st, RhsF16{(const char *)lv.in.p[0], lv.in.norm}, hl, s, lane);
-  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, rm < a.count ? rm : a.count - 1);
-  // out = (m + 4 + i tm g5) psi - 1/2 H psi
-  Cx<float> psi[12];
-  rv.in.load(s, psi);
   const float mp4 = (float)a.mp4, tm = (float)a.tm;
-  Cx<float> o[12];
+  mma_march<false>(a, 0u, in_all, hl, out_all, b_all,
+                   [&](const Cx<float> acc[4][3], const Plane &centre, unsigned s, unsigned, const RhsView<QCD_HIP_F16> &v,
+                       bool store) {
+                     // out = (m + 4 + i tm g5) psi - 1/2 H psi
+                     Cx<float> psi[12];
+                     SpinorIn<QCD_HIP_F16>::convert(centre.r, centre.n, psi);
+                     Cx<float> o[12];
 #pragma unroll
-  for (int sp = 0; sp < 4; sp++) {
-    const float g = sp < 2 ? tm : -tm;
+                     for (int sp = 0; sp < 4; sp++) {
+                       const float g = sp < 2 ? tm : -tm;
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const Cx<float> v = psi[3 * sp + c];
-      o[3 * sp + c] = Cx<float>{fmaf(-0.5f, acc[sp][c].re, fmaf(mp4, v.re, -(g * v.im))),
-                                fmaf(-0.5f, acc[sp][c].im, fmaf(mp4, v.im, g * v.re))};
-    }
-  }
-  apply_epilogue<EPI>(o, rv.b, s);
-  if (rm < a.count && s0 < a.V) rv.out.template store<true>(s, o);
+                       for (int c = 0; c < 3; c++) {
+                         const Cx<float> x = psi[3 * sp + c];
+                         o[3 * sp + c] = Cx<float>{fmaf(-0.5f, acc[sp][c].re, fmaf(mp4, x.re, -(g * x.im))),
+                                                   fmaf(-0.5f, acc[sp][c].im, fmaf(mp4, x.im, g * x.re))};
+                       }
+                     }
+                     apply_epilogue<EPI>(o, v.b, s);
+                     if (store) v.out.template store<true>(s, o);
+                   });
 }
 
-// Even-odd hop of an FP16 batch: output sites of parity e.par at
-// half-lattice entries 16 blk + b, neighbours read from the half field of the
-// other parity; epilogue M^-1 H in (EO_MINV) or M b - H in (EO_SCHUR).
+// Even-odd hop of an FP16 batch: output sites of parity e.par, neighbours
+// read from the half field of the other parity; epilogue M^-1 H in (EO_MINV)
+// or M b - H in (EO_SCHUR).
 template <int EPI>
 __global__ void __launch_bounds__(kBlock) k_eo_hop_mma(EoArgs e, SpinorIn<QCD_HIP_F16> in_all, HopLinks hl,
                                                        SpinorOut<QCD_HIP_F16> out_all, SpinorIn<QCD_HIP_F16> b_all) {
-  const DslashArgs &a = e.d;
-  const MmaTile t = mma_tile(a.nblk, a.count, a.rgroup);
-  if (!t.valid) return;   // whole wavefronts only
-  const unsigned lane = threadIdx.x % 64;
-  const unsigned hl0 = 16 * t.blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
-  const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
-  const Site st = eo_site(hl0 < e.Vh ? hl0 : e.Vh - 1, e.par, a);
-  const unsigned h0 = 16 * t.blk + (lane >> 2), rm = 4 * t.group + (lane & 3u);
-  const unsigned h = h0 < e.Vh ? h0 : e.Vh - 1;
-  // the MFMA lane's full-lattice site: that of load lane b (row 0 of the group)
-  const unsigned sm = pull(st.s, (int)(4u * (lane >> 2)));
-  Cx<float> acc[4][3];
-  mma_hopping<true>(acc, a, st, RhsF16{(const char *)lv.in.p[0], lv.in.norm}, hl, sm, lane);
-  const RhsView<QCD_HIP_F16> rv = rhs_view(a, in_all, out_all, b_all, rm < a.count ? rm : a.count - 1);
-  Cx<float> o[12];
-  eo_epilogue<EPI>(o, acc, e, rv.b, h);
-  if (rm < a.count && h0 < e.Vh) rv.out.template store<true>(h, o);
+  mma_march<true>(e.d, e.par, in_all, hl, out_all, b_all,
+                  [&](const Cx<float> acc[4][3], const Plane &, unsigned, unsigned h, const RhsView<QCD_HIP_F16> &v,
+                      bool store) {
+                    Cx<float> o[12];
+                    eo_epilogue<EPI>(o, acc, e, v.b, h);
+                    if (store) v.out.template store<true>(h, o);
+                  });
 }
 
 // build the hop links of FP16 gauge field g for the phases of lattice lat
@@ -440,18 +484,64 @@ int mma_prepare(qcd_hip_field g, qcd_hip_lattice lat, hipStream_t st) {
 }
 
 // rhs groups per super-group (mma_tile); QCD_HIP_MMA_RGROUP overrides the
-// default, all groups
+// default (0 = all groups)
 static unsigned mma_rgroup() {
   static const unsigned r = [] {
     const char *e = getenv("QCD_HIP_MMA_RGROUP");
-    return e ? (unsigned)std::atoi(e) : 0u;
+    return e ? (unsigned)std::atoi(e) : 3u;
   }();
   return r;
 }
 
-// workgroups of kBlock / 64 wavefronts over (site blocks of 16) x (rhs groups of 4)
-static unsigned mma_grid(unsigned nblk, unsigned count) {
-  const unsigned waves = nblk * ((count + 3) / 4), per = kBlock / 64;
+// z steps per wavefront (mma_march); QCD_HIP_MMA_ZLEN overrides the default
+static unsigned mma_zlen(unsigned Lz) {
+  static const unsigned r = [] {
+    const char *e = getenv("QCD_HIP_MMA_ZLEN");
+    return e ? (unsigned)std::atoi(e) : 12u;
+  }();
+  return r < 1 || r > Lz ? Lz : r;
+}
+
+// x-rows per x-tile of the block order (tile_block); QCD_HIP_MMA_XTILE
+// overrides the default, 0 = index order
+static unsigned mma_xtile() {
+  static const unsigned r = [] {
+    const char *e = getenv("QCD_HIP_MMA_XTILE");
+    return e ? (unsigned)std::atoi(e) : 4u;
+  }();
+  return r;
+}
+
+// the tiled block order for rows of L0p columns (EO: half columns): units of
+// whole x-rows that 16-column blocks fill exactly, as many units per x-tile
+// as fit in mma_xtile() rows and divide the x-line; index order when the
+// x extent is not a whole number of units
+static void mma_block_order(DslashArgs &a, unsigned L0p) {
+  a.tile_bpu = a.tile_wu = a.tile_upr = 0;
+  const unsigned X = mma_xtile();
+  if (X == 0 || L0p == 0) return;
+  const unsigned g = std::gcd(L0p, 16u), span = L0p / g * 16;   // lcm(L0p, 16) columns
+  const unsigned rows = span / L0p, bpu = span / 16;
+  if (a.L[1] % rows) return;
+  const unsigned upr = a.L[1] / rows;
+  unsigned wu = 1;
+  for (unsigned w = 1; w <= upr; w++)
+    if (upr % w == 0 && w * rows <= X) wu = w;
+  a.tile_bpu = bpu;
+  a.tile_wu = wu;
+  a.tile_upr = upr;
+}
+
+// the z-march fields of a for PP columns per plane (rows of L0p columns);
+// returns the grid: workgroups of kBlock / 64 wavefronts over column blocks x
+// z chunks x rhs groups
+static unsigned mma_setup(DslashArgs &a, unsigned PP, unsigned L0p) {
+  a.nblk = (PP + 15) / 16;
+  a.zlen = mma_zlen(a.L[3]);
+  a.rgroup = mma_rgroup();
+  mma_block_order(a, L0p);
+  const unsigned nzc = (a.L[3] + a.zlen - 1) / a.zlen;
+  const unsigned waves = a.nblk * nzc * ((a.count + 3) / 4), per = kBlock / 64;
   return (waves + per - 1) / per;
 }
 
@@ -461,9 +551,7 @@ int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_
     set_error("dslash: the matrix-core traversal needs FP16 spinors and links and the D or Q epilogue");
     return 1;
   }
-  a.nblk = (unsigned)((a.V + 15) / 16);
-  a.rgroup = mma_rgroup();
-  const unsigned grid = mma_grid(a.nblk, a.count);
+  const unsigned grid = mma_setup(a, a.L[0] * a.L[1] * a.L[2], a.L[0]);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
   const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);
@@ -482,9 +570,11 @@ int eo_mma_launch(int epi, const EoArgs &e0, qcd_hip_field out, int ro, qcd_hip_
     return 1;
   }
   EoArgs e = e0;
-  e.d.nblk = (e.Vh + 15) / 16;
-  e.d.rgroup = mma_rgroup();
-  const unsigned grid = mma_grid(e.d.nblk, e.d.count);
+  if (e.d.L[0] % 2) {
+    set_error("eo: the matrix-core hop needs an even time extent");
+    return 1;
+  }
+  const unsigned grid = mma_setup(e.d, e.d.L[0] * e.d.L[1] * e.d.L[2] / 2, e.d.L[0] / 2);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
   const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);

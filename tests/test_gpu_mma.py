@@ -96,11 +96,12 @@ def test_mma_dslash_config_sizes(gpu, L, nrhs):
     Q.dslash_tune(y, x, g)   # leave a tuned configuration behind
 
 
-@pytest.mark.parametrize("nrhs", [1, 5, 12])
+@pytest.mark.parametrize("nrhs", [16, 21])
 def test_mma_even_odd_schur(gpu, nrhs):
-    """The FP16 Schur operator M - H_oe M^-1 H_eo (both hops on the matrix
-    cores) rhs by rhs against the oracle's full operator, and a batched rhs
-    equal to the same rhs alone."""
+    """The FP16 Schur operator M - H_oe M^-1 H_eo of a batch of 16 or more rhs
+    (both hops on the matrix cores; 21: a ragged last group of 4) rhs by rhs
+    against the oracle's full operator, and the bits of each rhs equal to those
+    of the same rhs in another matrix-core batch."""
     Q = gpu
     L, theta = (8, 4, 6, 4), (np.pi, 0.2, 0.0, -0.6)
     m, tm = 1 / (2 * 0.13) - 4, 0.05
@@ -112,14 +113,20 @@ def test_mma_even_odd_schur(gpu, nrhs):
     g = Q.Gauge(lat, Q.F16)
     Q.convert(g, g64)
     vs = [Q.eo_split(Q.spinor_gaussian(L, 790 + r), L)[1] for r in range(nrhs)]
-    xb = Q.Spinor(half, Q.F16, nrhs=nrhs)
+
+    def schur(vecs):
+        xb = Q.Spinor(half, Q.F16, nrhs=len(vecs))
+        for r, v in enumerate(vecs):
+            xb.upload(v, rhs=r)
+        yb = Q.Spinor(half, Q.F16, nrhs=len(vecs))
+        Q.dslash_schur(yb, xb, g, m, tm)
+        return [yb.download(r) for r in range(len(vecs))]
+
+    ys = schur(vs)
     for r, v in enumerate(vs):
-        xb.upload(v, rhs=r)
-    yb = Q.Spinor(half, Q.F16, nrhs=nrhs)
-    Q.dslash_schur(yb, xb, g, m, tm)
-    for r, v in enumerate(vs):
-        assert rel_l2(yb.download(r), schur_reference(Q, L, u, v, m, tm, theta)) <= 4e-3, r
-    x1 = Q.Spinor(half, Q.F16).upload(vs[-1])
-    y1 = Q.Spinor(half, Q.F16)
-    Q.dslash_schur(y1, x1, g, m, tm)
-    assert np.array_equal(yb.download(nrhs - 1), y1.download())
+        assert rel_l2(ys[r], schur_reference(Q, L, u, v, m, tm, theta)) <= 4e-3, r
+    # the same vectors in reverse order, in a batch of 17 (rows move between
+    # groups of 4 and between lanes)
+    rev = schur(vs[::-1][:17])
+    for k, y in enumerate(rev):
+        assert np.array_equal(y, ys[nrhs - 1 - k]), k
