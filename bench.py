@@ -38,13 +38,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cores_allowed():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return None
+
+
+# Read before anything starts OpenMP: with OMP_PROC_BIND set, the runtime binds
+# the main thread to one core, and the affinity mask read afterwards is that
+# one core.
+CORES_ALLOWED = _cores_allowed()
+
+
 def cpu_threads():
     """Threads for the CPU baseline: every core this process may run on, at
     most 16 (the host CPU share of one GPU on the benchmark machines)."""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
+    n = CORES_ALLOWED or os.cpu_count() or 1
     return max(1, min(16, n))
 
 
@@ -59,11 +69,7 @@ def host_info():
                     break
     except OSError:
         pass
-    try:
-        allowed = len(os.sched_getaffinity(0))
-    except AttributeError:
-        allowed = None
-    return {"nproc": os.cpu_count(), "cores_allowed": allowed, "cpu_model": model}
+    return {"nproc": os.cpu_count(), "cores_allowed": CORES_ALLOWED, "cpu_model": model, "threads": cpu_threads()}
 
 
 def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
@@ -85,12 +91,16 @@ def dslash_leg(Q, L, prec, steps, warmup, seed=3001):
     Q.synchronize()
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
     gbps = V * BYTES_PER_SITE[prec] / (ms * 1e-3) / 1e9
-    return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS,
-            "config": {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march over LDS-staged row tiles",
-                                      "z-march, forward z neighbour carried in LDS", "z-march, two rhs per lane",
-                                      "matrix cores (MFMA), 16 sites x 4 rhs per wavefront"][cfg[0] & 0xff],
-                       "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
-                       "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}}
+    return {"ms": ms, "GBps": gbps, "frac": gbps / HBM_PEAK_GBPS, "config": describe_config(cfg)}
+
+
+def describe_config(cfg):
+    """The launch configuration (map, occ) the autotuner chose, in words."""
+    return {"traversal": ["xcd-contiguous", "xcd-y-slab", "z-march", "z-march over LDS-staged row tiles",
+                          "z-march, forward z neighbour carried in LDS", "z-march, two rhs per lane",
+                          "matrix cores (MFMA), z-march of 16 columns x 4 rhs per wavefront"][cfg[0] & 0xff],
+            "z_steps_per_lane": cfg[0] >> 8, "min_waves_per_simd": cfg[1] & 15,
+            "sched_fence": bool(cfg[1] & 16), "map": cfg[0], "occ": cfg[1]}
 
 
 def multi_rhs_dslash_leg(Q, L, prec, nrhs, steps, warmup, seed=3101):
@@ -112,12 +122,52 @@ def multi_rhs_dslash_leg(Q, L, prec, nrhs, steps, warmup, seed=3101):
         x.upload(psi, rhs=r)
     y = Q.Spinor(lat, p, nrhs=nrhs)
     m = 1 / (2 * 0.132) - 4
-    Q.dslash_tune(y, x, g)
+    tuned = Q.dslash_tune(y, x, g)
     ms = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
     spinor, links = {"f64": (192, 576), "f32": (96, 288), "f16": (52, 144)}[prec]
     bps = nrhs * 2 * spinor + links
-    return {"rhs": nrhs, "ms": ms, "us_per_rhs": ms * 1e3 / nrhs, "bytes_per_site": bps,
-            "GBps": V * bps / (ms * 1e-3) / 1e9}
+    out = {"rhs": nrhs, "ms": ms, "us_per_rhs": ms * 1e3 / nrhs, "bytes_per_site": bps,
+           "GBps": V * bps / (ms * 1e-3) / 1e9, "tuned": describe_config((tuned[0], tuned[1]))["traversal"]}
+    if prec == "f16":
+        # both batched FP16 kernels, whichever the tuner chose: the matrix cores
+        # (16 sites x 4 rhs per wavefront) and the vector ALUs (two rhs per lane)
+        for key, map_, occ in (("mfma", 6, 0), ("valu_two_rhs", 5 | (6 << 8), 16)):
+            Q.dslash_set_config(x, g, map_, occ)
+            t = Q.dslash_time(y, x, g, m, 0.0, Q.OP_D, warmup=warmup, iters=steps)
+            out[key] = {"us_per_rhs": t * 1e3 / nrhs, "GBps": V * bps / (t * 1e-3) / 1e9}
+        Q.dslash_set_config(x, g, tuned[0], tuned[1])
+    return out
+
+
+def schur_leg(Q, L, nrhs, steps, seed=3201):
+    """The FP16 even-odd Schur operator M - H_oe M^-1 H_eo of the batched inner
+    solve over nrhs odd-site half vectors (two hop launches; from 16 rhs up on
+    the matrix cores).  Algorithmic bytes per half site and rhs: each hop reads
+    and writes a half spinor (2 x 52 B) and the second also reads b (52 B);
+    the links (144 B per site) are read once per hop launch."""
+    import time
+    lat = Q.Lattice(L)
+    half = Q.Lattice((L[0] // 2,) + tuple(L[1:]))
+    g64 = Q.Gauge(lat)
+    Q.gauge_generate_device(g64, 0.3, seed)
+    g = Q.Gauge(lat, Q.F16)
+    Q.convert(g, g64)
+    x = Q.Spinor(half, Q.F16, nrhs=nrhs)
+    for r in range(nrhs):
+        x.upload(Q.spinor_gaussian((half.V, 1, 1, 1), seed + 1 + r), rhs=r)
+    y = Q.Spinor(half, Q.F16, nrhs=nrhs)
+    m = 1 / (2 * 0.132) - 4
+    for _ in range(3):
+        Q.dslash_schur(y, x, g, m)
+    Q.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        Q.dslash_schur(y, x, g, m)
+    Q.synchronize()
+    el = (time.perf_counter() - t) / steps
+    bps = nrhs * 5 * 52 + 2 * 144
+    return {"rhs": nrhs, "ms": el * 1e3, "us_per_rhs": el * 1e6 / nrhs, "bytes_per_half_site": bps,
+            "GBps": half.V * bps / el / 1e9}
 
 
 def pmc_traffic(L, prec, cfg, timeout=150):
@@ -397,7 +447,11 @@ def main():
         mr = multi_rhs_dslash_leg(Q, L, "f16", nr, max(10, args.steps // 10), max(2, args.warmup // 10))
         dsl[f"f16_{nr}rhs"] = mr
         log(f"f16 dslash 24^3x48, {nr} rhs: {mr['us_per_rhs']:.1f} us per rhs, {mr['GBps']:.0f} GB/s "
-            f"({mr['bytes_per_site']} B/site)")
+            f"({mr['bytes_per_site']} B/site, {mr['tuned']}); MFMA {mr['mfma']['us_per_rhs']:.1f}, "
+            f"two rhs per lane {mr['valu_two_rhs']['us_per_rhs']:.1f} us per rhs")
+    sc = schur_leg(Q, L, 48, max(10, args.steps // 10))
+    dsl["f16_schur_48rhs"] = sc
+    log(f"f16 even-odd Schur operator 24^3x48, 48 rhs: {sc['us_per_rhs']:.1f} us per rhs, {sc['GBps']:.0f} GB/s")
 
     traffic, traffic_note = None, "not measured (--no-pmc)"
     if not args.no_pmc:

@@ -771,8 +771,10 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
     }
   }
   // Batches (FP32/FP16 fields with several rhs): the best one-rhs traversal
-  // against two rhs per lane, timed over the first min(nrhs, 16) rhs.
-  const int cnt = std::min(std::min(in->nrhs, out->nrhs), 16);
+  // against two rhs per lane and the matrix cores, timed over the first
+  // min(nrhs, 48) rhs (at 24^3x48 the matrix cores trail at 16 rhs and lead
+  // at 48, so a choice made on fewer rhs than the batch can be wrong).
+  const int cnt = std::min(std::min(in->nrhs, out->nrhs), 48);
   if (cnt >= 2 && in->prec != QCD_HIP_F64) {
     const Batch bt{nullptr, cnt, 1, 1, 1};
     float bbest = 1e30f;
