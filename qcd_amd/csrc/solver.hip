@@ -325,20 +325,21 @@ __global__ void __launch_bounds__(1024) k_scal_rnorm(GcrScal *scs, const double 
   }
 }
 
-// res_0 = r / |r| in the inner precision
-template <int P>
-__global__ void __launch_bounds__(kBlock) k_prepare_src(SpinorSet<QCD_HIP_F64> r, const GcrScal *sc, SpinorSet<P> Rs,
-                                                        int n, const int *ids, unsigned V) {
+// res_0 = r / |r| in the inner precision P (r in precision PI)
+template <int PI, int P>
+__global__ void __launch_bounds__(kBlock) k_prepare_src(SpinorSet<PI> r, const GcrScal *sc, SpinorSet<P> Rs, int n,
+                                                        const int *ids, unsigned V) {
   using R = typename Prec<P>::R;
+  using RI = typename Prec<PI>::R;
   const int b = ids[blockIdx.y];
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
   const double f = sc[b].rscale;
-  Cx<double> v[12];
+  Cx<RI> v[12];
   r.load(b, s, v);
   Cx<R> o[12];
 #pragma unroll
-  for (int c = 0; c < 12; c++) o[c] = Cx<R>{(R)(v[c].re * f), (R)(v[c].im * f)};
+  for (int c = 0; c < 12; c++) o[c] = Cx<R>{(R)((double)v[c].re * f), (R)((double)v[c].im * f)};
   Rs.store(b * n, s, o);
 }
 
@@ -411,7 +412,10 @@ struct qcd_hip_solver_s {
   qcd_hip_lattice half = nullptr;    // half lattice of the inner vectors (owned)
   unsigned Vi = 0, nbi = 0;          // inner vector sites and site-kernel workgroups
   qcd_hip_field te = nullptr;        // EO: B inner even temporaries of the matvec
-  qcd_hip_field re64 = nullptr, ro64 = nullptr, t64 = nullptr, eo64 = nullptr;   // EO: B FP64 half vectors each
+  // EO: B half vectors each for the outer step: even and odd residual halves,
+  // M^-1 temporaries, the odd correction; FP64 for an FP64 inner solve, the
+  // inner precision otherwise
+  qcd_hip_field reh = nullptr, roh = nullptr, th = nullptr, eoh = nullptr;
   // the captured GCR cycle (prm.graph) and the launch arguments it was
   // captured with; every buffer it touches belongs to the solver
   hipGraphExec_t cycle = nullptr;
@@ -496,8 +500,8 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
   const dim3 g2(S->nbi, (unsigned)count);
   const unsigned V = S->Vi;
   if (!S->prm.eo) {
-    k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc, SpinorSet<P>::of(S->Rs),
-                                            S->prm.n_restart, S->ids, V);
+    k_prepare_src<QCD_HIP_F64, P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->r64), S->sc,
+                                                         SpinorSet<P>::of(S->Rs), S->prm.n_restart, S->ids, V);
     QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
     k_accumulate<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(x), x0, SpinorSet<P>::of(S->E), S->sc, S->ids,
                                            V);
@@ -506,20 +510,41 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
   }
   // Even-odd: D e = r  <=>  D^ e_o = r_o - H_oe M^-1 r_e,  e_e = M^-1 (r_e - H_eo e_o)
   const double m = 1.0 / (2.0 * kappa) - 4.0;
+  const size_t sstride = sizeof(GcrScal) / sizeof(double);
   int grid = 0;
-  QH_TRY(eo_split(S->r64, 0, S->re64, S->ro64, S->ids, count, st));
-  QH_TRY(eo_minv(S->lat, S->re64, S->t64, m, mu, S->ids, count, st));
-  QH_TRY(eo_launch(EO_SUB, true, S->eo64, 0, 1, S->t64, 0, 1, S->g64, S->ro64, 0, 1, S->ids, count, m, mu, 1, S->part,
+  if constexpr (P == QCD_HIP_F64) {
+    QH_TRY(eo_split(S->r64, 0, S->reh, S->roh, S->ids, count, st));
+    QH_TRY(eo_minv(S->lat, S->reh, S->th, m, mu, S->ids, count, st));
+    QH_TRY(eo_launch(EO_SUB, true, S->eoh, 0, 1, S->th, 0, 1, S->g64, S->roh, 0, 1, S->ids, count, m, mu, 1, S->part,
+                     st, &grid));
+    k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);   // |r^_o| for the scaling
+    k_prepare_src<QCD_HIP_F64, P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->eoh), S->sc,
+                                                         SpinorSet<P>::of(S->Rs), S->prm.n_restart, S->ids, V);
+    QH_CHECK(hipGetLastError());
+    QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
+    QH_TRY(eo_scale64(S->lat, S->E, S->eoh, &S->sc[0].rnorm, sstride, S->ids, count, st));
+    QH_TRY(eo_launch(EO_SUB_MINV, false, S->th, 0, 1, S->eoh, 0, 1, S->g64, S->reh, 0, 1, S->ids, count, m, mu, 0,
+                     nullptr, st, nullptr));
+    QH_TRY(eo_accumulate(x, x0, S->th, S->eoh, S->ids, count, st));
+    return 0;
+  }
+  // Mixed precision: only the true residual, the accumulation into x and the
+  // reductions are FP64.  The halves of the residual, the inner source
+  // r^_o and the even correction e_e are formed in the inner precision with
+  // the inner-precision links: their rounding only perturbs the correction
+  // of this outer step, which the next FP64 true residual measures.
+  QH_TRY(eo_split_minv(S->r64, 0, S->reh, S->roh, S->th, m, mu, S->ids, count, st));
+  QH_TRY(eo_launch(EO_SUB, true, S->eoh, 0, 1, S->th, 0, 1, S->gin, S->roh, 0, 1, S->ids, count, m, mu, 1, S->part,
                    st, &grid));
   k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);   // |r^_o| for the scaling
-  k_prepare_src<P><<<g2, kBlock, 0, st>>>(SpinorSet<QCD_HIP_F64>::of(S->eo64), S->sc, SpinorSet<P>::of(S->Rs),
-                                          S->prm.n_restart, S->ids, V);
+  k_prepare_src<P, P><<<g2, kBlock, 0, st>>>(SpinorSet<P>::of(S->eoh), S->sc, SpinorSet<P>::of(S->Rs),
+                                             S->prm.n_restart, S->ids, V);
   QH_CHECK(hipGetLastError());
   QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
-  QH_TRY(eo_scale64(S->lat, S->E, S->eo64, &S->sc[0].rnorm, sizeof(GcrScal) / sizeof(double), S->ids, count, st));
-  QH_TRY(eo_launch(EO_SUB_MINV, false, S->t64, 0, 1, S->eo64, 0, 1, S->g64, S->re64, 0, 1, S->ids, count, m, mu, 0,
+  QH_TRY(eo_scale(S->lat, S->E, &S->sc[0].rnorm, sstride, S->ids, count, st));   // e_o = |r^_o| E
+  QH_TRY(eo_launch(EO_SUB_MINV, false, S->th, 0, 1, S->E, 0, 1, S->gin, S->reh, 0, 1, S->ids, count, m, mu, 0,
                    nullptr, st, nullptr));
-  QH_TRY(eo_accumulate(x, x0, S->t64, S->eo64, S->ids, count, st));
+  QH_TRY(eo_accumulate_p(x, x0, S->th, S->E, S->ids, count, st));
   return 0;
 }
 
@@ -630,7 +655,7 @@ size_t solver_device_bytes(const int L[4], const qcd_hip_solver_params *prm) {
   const long sti = round_up(Vi, kTile);
   size_t b = 0;
   b += 2 * field(sti, ip, n * B) + field(sti, ip, B);                  // Rs, Ws, E
-  if (prm->eo) b += field(sti, ip, B) + 4 * field(sti, QCD_HIP_F64, B); // te, re64, ro64, t64, eo64
+  if (prm->eo) b += 5 * field(sti, ip, B);                             // te, reh, roh, th, eoh
   b += 2 * field(lat.stride, QCD_HIP_F64, B);                          // r64, bq
   b += sizeof(GcrScal) * B + sizeof(double) * B + sizeof(int) * B;      // sc, rn, ids
   const size_t nb = (size_t)grid_for(V);
@@ -684,8 +709,8 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   if (!rc) rc = qcd_hip_spinor_create(&S->Ws, ilat, prm->inner_prec, n * B);
   if (!rc) rc = qcd_hip_spinor_create(&S->E, ilat, prm->inner_prec, B);
   if (!rc && prm->eo) rc = qcd_hip_spinor_create(&S->te, ilat, prm->inner_prec, B);
-  for (qcd_hip_field *f : {&S->re64, &S->ro64, &S->t64, &S->eo64})
-    if (!rc && prm->eo) rc = qcd_hip_spinor_create(f, ilat, QCD_HIP_F64, B);
+  for (qcd_hip_field *f : {&S->reh, &S->roh, &S->th, &S->eoh})
+    if (!rc && prm->eo) rc = qcd_hip_spinor_create(f, ilat, prm->inner_prec, B);
   if (!rc) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, B);
   if (!rc) rc = qcd_hip_spinor_create(&S->bq, lat, QCD_HIP_F64, B);
   if (!rc) rc = dev_alloc((void **)&S->sc, sizeof(GcrScal) * B);
@@ -718,7 +743,7 @@ void qcd_hip_solver_destroy(qcd_hip_solver S) {
   qcd_hip_field_destroy(S->E);
   qcd_hip_field_destroy(S->r64);
   qcd_hip_field_destroy(S->bq);
-  for (qcd_hip_field f : {S->te, S->re64, S->ro64, S->t64, S->eo64}) qcd_hip_field_destroy(f);
+  for (qcd_hip_field f : {S->te, S->reh, S->roh, S->th, S->eoh}) qcd_hip_field_destroy(f);
   if (S->gin && S->gin != S->g64) qcd_hip_field_destroy(S->gin);
   if (S->stream) {
     (void)hipStreamSynchronize(S->stream);
