@@ -49,6 +49,33 @@ struct SpinorSet {
       for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
     }
   }
+  // store v and leave in v the stored values (FP16: rounded to the stored
+  // half form and multiplied back by the site norm), without reading them
+  __device__ __forceinline__ void store_back(int j, unsigned s, Cx<R> v[12]) const {
+    if constexpr (P == QCD_HIP_F16) {
+      T *b = base + (size_t)j * vstride;
+      const unsigned off = SpinorIn<P>::offset(s);
+      R mx = R(0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+      if (!(mx >= FLT_MIN)) mx = R(0);
+      const R inv = mx > R(0) ? R(1) / mx : R(0);
+      norm[(size_t)j * nstride + s] = mx;
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+        u32x4 q;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
+          const Cx<R> c = to_cx<R>(as_half2(q[i]));
+          v[4 * g + i] = Cx<R>{c.re * mx, c.im * mx};
+        }
+        *(u32x4 *)((char *)b + g * 4 * kTile * 4 + off) = q;
+      }
+    } else {
+      store(j, s, v);
+    }
+  }
   __device__ __forceinline__ void store(int j, unsigned s, const Cx<R> v[12]) const {
     T *b = base + (size_t)j * vstride;
     const unsigned off = SpinorIn<P>::offset(s);

@@ -155,8 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_norm(SpinorSet<P> W, Spino
       }
 #pragma unroll
       for (int c = 0; c < 12; c++) wk[c] = Cx<R>{(R)acc[c].re, (R)acc[c].im};
-      W.store(b * n + k, s, wk);
-      if constexpr (P == QCD_HIP_F16) W.load(b * n + k, s, wk);   // norms of the stored (scaled) vector
+      W.store_back(b * n + k, s, wk);   // the norms below are those of the stored (rounded) vector
     }
     Cx<R> sv[12];
     Rs.load(b * n, s, sv);
