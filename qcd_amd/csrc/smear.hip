@@ -21,6 +21,7 @@
 // of sites run together on one XCD and share its links through that XCD's L2.
 // The whole-volume form (t < 0) is the same with T slices per pass; its
 // compact order is the field's own.
+#include <type_traits>
 #include "blas_common.h"
 
 namespace qcdhip {
@@ -31,6 +32,8 @@ struct GaussArgs {
   unsigned t0;           // the slice when tn == 1
   unsigned n;            // sites per right-hand side: tn X Y Z
   unsigned count;        // right-hand sides in the launch
+  unsigned ncb;          // k_gauss: workgroups of columns per z chunk
+  unsigned zlen;         // k_gauss: z steps per lane
 };
 
 // lane -> (row, compact site j); j = tt + tn r with r = x + X(y + Yz)
@@ -70,56 +73,115 @@ __device__ __forceinline__ unsigned field_of(const GaussArgs &g, const GaussSite
 // One iteration: input from a compact buffer, output to a compact buffer
 // (OUT_FIELD false) or to the field (OUT_FIELD true).  U holds the links in
 // compact order.
+//
+// z-march (as the Dslash's): a lane owns one (t, x, y) column of one rhs and
+// steps through zlen planes.  It carries the centre spinor of the next step
+// (loaded as this step's forward z neighbour) and the backward z hop
+// U_z(z-1)^+ psi(z-1) (made while it held psi(z-1) and U_z(z-1)), so a step
+// reads only the z+1 plane, the x and y neighbours (the neighbouring lanes'
+// centres, read in the same step) and the links.  Hops are added into acc
+// one at a time and pinned there.  acc = psi / alpha + sum of hops and
+// out = alpha nrm acc; alpha = 0 never reaches the kernel (the step is the
+// identity then).  Rows are the fastest block index: the rows of a column
+// block read its links through one L2.
 template <int P, bool OUT_FIELD>
 __global__ void __launch_bounds__(kBlock) k_gauss(GaussArgs g, SpinorIn<P> in, GaugeIn<P> U, SpinorOut<P> out,
                                                   double alpha, double nrm) {
   using R = typename Prec<P>::R;
-  const GaussSite q = gauss_site(g);
-  if (!q.valid) return;
-  const SpinorIn<P> src = in.advanced(q.row);
-  const SpinorOut<P> dst = out.advanced(q.row);
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned row = lb % g.count, rest = lb / g.count;
+  const unsigned zc = rest / g.ncb, col = (rest - zc * g.ncb) * kBlock + threadIdx.x;
+  const unsigned PC = g.tn * g.X * g.Y;   // columns per plane
+  const unsigned z0 = zc * g.zlen;
+  if (col >= PC || z0 >= g.Z) return;
+  const unsigned z1 = min(z0 + g.zlen, g.Z);
+  GaussSite q;
+  q.row = row;
+  q.tt = col % g.tn;
+  {
+    const unsigned rxy = col / g.tn;
+    q.y = rxy / g.X;
+    q.x = rxy - q.y * g.X;
+  }
+  q.valid = true;
+  const SpinorIn<P> src = in.advanced(row);
+  const SpinorOut<P> dst = out.advanced(row);
   const unsigned XY = g.X * g.Y;
-  const unsigned c = compact_of(g, q, q.r);
-  Cx<R> acc[12];
+  const R ia = (R)(1.0 / alpha), f = (R)(alpha * nrm);
+  auto pin = [&](Cx<R> v[12]) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) acc[k] = Cx<R>{R(0), R(0)};
-  Cx<R> psi[12], u[9], y[3];
-  // rf, rb: forward and backward slice points in direction nu
-  auto step = [&](unsigned rf, unsigned rb, int nu) {
-    Cx<R> up[12], ud[12];
-    src.load(compact_of(g, q, rf), psi);
-    U.load(nu, c, u);
-#pragma unroll
-    for (int sp = 0; sp < 4; sp++) {
-      su3_mul<false>(y, u, &psi[3 * sp]);
-#pragma unroll
-      for (int c = 0; c < 3; c++) up[3 * sp + c] = y[c];
-    }
-    const unsigned cb = compact_of(g, q, rb);
-    src.load(cb, psi);
-    U.load(nu, cb, u);
-#pragma unroll
-    for (int sp = 0; sp < 4; sp++) {
-      su3_mul<true>(y, u, &psi[3 * sp]);
-#pragma unroll
-      for (int c = 0; c < 3; c++) ud[3 * sp + c] = y[c];
-    }
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      acc[k].re += up[k].re + ud[k].re;
-      acc[k].im += up[k].im + ud[k].im;
-    }
+    for (int k = 0; k < 12; k++) asm volatile("" : "+v"(v[k].re), "+v"(v[k].im));
+    __builtin_amdgcn_sched_barrier(0);
   };
-  const unsigned r = q.r;
-  step(q.x == g.X - 1 ? r - (g.X - 1) : r + 1, q.x == 0 ? r + (g.X - 1) : r - 1, 1);
-  step(q.y == g.Y - 1 ? r - (g.Y - 1) * g.X : r + g.X, q.y == 0 ? r + (g.Y - 1) * g.X : r - g.X, 2);
-  step(q.z == g.Z - 1 ? r - (g.Z - 1) * XY : r + XY, q.z == 0 ? r + (g.Z - 1) * XY : r - XY, 3);
-  src.load(c, psi);
-  const R al = (R)alpha, nr = (R)nrm;
+  // acc += W psi(rr) with W = U_nu(c) (forward) or U_nu(rr)^+ (backward)
+  auto hop = [&](Cx<R> acc[12], unsigned rr, unsigned c, int nu, auto dag) {
+    Cx<R> ps[12], u[9], y[3];
+    const unsigned cc = compact_of(g, q, rr);
+    src.load(cc, ps);
+    U.load(nu, decltype(dag)::value ? cc : c, u);
 #pragma unroll
-  for (int k = 0; k < 12; k++)
-    psi[k] = Cx<R>{(psi[k].re + acc[k].re * al) * nr, (psi[k].im + acc[k].im * al) * nr};
-  dst.store(OUT_FIELD ? field_of(g, q, r) : c, psi);
+    for (int sp = 0; sp < 4; sp++) {
+      su3_mul<decltype(dag)::value>(y, u, &ps[3 * sp]);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        acc[3 * sp + k].re += y[k].re;
+        acc[3 * sp + k].im += y[k].im;
+      }
+    }
+    pin(acc);
+  };
+  // the carried state: centre psi(z) and bz = U_z(z-1)^+ psi(z-1)
+  Cx<R> cen[12], bz[12];
+  {
+    const unsigned zm = z0 == 0 ? g.Z - 1 : z0 - 1;
+    const unsigned rm = q.x + g.X * (q.y + g.Y * zm);
+#pragma unroll
+    for (int k = 0; k < 12; k++) bz[k] = Cx<R>{R(0), R(0)};
+    hop(bz, rm, 0u, 3, std::true_type{});
+    src.load(compact_of(g, q, q.x + g.X * (q.y + g.Y * z0)), cen);
+  }
+  for (unsigned z = z0; z < z1; z++) {
+    q.z = z;
+    const unsigned r = q.x + g.X * (q.y + g.Y * z);
+    q.r = r;
+    const unsigned c = compact_of(g, q, r);
+    Cx<R> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = Cx<R>{cen[k].re * ia + bz[k].re, cen[k].im * ia + bz[k].im};
+    pin(acc);
+    hop(acc, q.x == g.X - 1 ? r - (g.X - 1) : r + 1, c, 1, std::false_type{});
+    hop(acc, q.x == 0 ? r + (g.X - 1) : r - 1, c, 1, std::true_type{});
+    hop(acc, q.y == g.Y - 1 ? r - (g.Y - 1) * g.X : r + g.X, c, 2, std::false_type{});
+    hop(acc, q.y == 0 ? r + (g.Y - 1) * g.X : r - g.X, c, 2, std::true_type{});
+    // last, the z direction: the next step's backward hop from this centre
+    // and U_z here (the centre dies with it), then the forward hop, whose
+    // spinor is the next step's centre
+    {
+      Cx<R> u[9], y[3];
+      U.load(3, c, u);
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) {
+        su3_mul<true>(y, u, &cen[3 * sp]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) bz[3 * sp + k] = y[k];
+      }
+      pin(bz);
+      const unsigned rf = q.z == g.Z - 1 ? r - (g.Z - 1) * XY : r + XY;
+      src.load(compact_of(g, q, rf), cen);
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) {
+        su3_mul<false>(y, u, &cen[3 * sp]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          acc[3 * sp + k].re += y[k].re;
+          acc[3 * sp + k].im += y[k].im;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = Cx<R>{acc[k].re * f, acc[k].im * f};
+    dst.store(OUT_FIELD ? field_of(g, q, r) : c, acc);
+  }
 }
 
 // Stored values (and FP16 norms) of the slice's sites: field -> compact buffer.
@@ -154,6 +216,18 @@ __global__ void __launch_bounds__(kBlock) k_gauss_gather_links(GaussArgs g, cons
   const size_t s = g.t0 + (size_t)g.T * r;
 #pragma unroll
   for (int k = 9; k < 36; k++) dst[gauge_index_e<E>(k, r, dstride)] = src[gauge_index_e<E>(k, s, sstride)];
+}
+
+// z steps per lane of k_gauss: four z chunks (at least 2 steps).  On a 48^3
+// slice with 12 rhs every length from 2 to 24 measured 201-226 us per
+// iteration (48: 328).  QCD_HIP_GAUSS_ZLEN overrides it.
+static unsigned gauss_zlen(unsigned Z) {
+  static const int env = [] {
+    const char *e = std::getenv("QCD_HIP_GAUSS_ZLEN");
+    return e ? std::atoi(e) : 0;
+  }();
+  const unsigned r = env > 0 ? (unsigned)env : std::max(2u, (Z + 3) / 4);
+  return r > Z ? Z : r;
 }
 
 // Rows of one pass: two compact buffers of `rows` right-hand sides each stay
@@ -211,9 +285,12 @@ static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, in
   }
   const GaugeIn<P> U = GaugeIn<P>::of(links);
   const unsigned nb = (unsigned)grid_for(g.n);
+  g.zlen = gauss_zlen(g.Z);
+  g.ncb = (unsigned)grid_for((long)g.tn * g.X * g.Y);
+  const unsigned nzc = (g.Z + g.zlen - 1) / g.zlen;
   for (int r0 = 0; r0 < v->nrhs; r0 += rows) {
     g.count = (unsigned)std::min(rows, v->nrhs - r0);
-    const unsigned grid = nb * g.count;
+    const unsigned grid = nb * g.count, zgrid = g.ncb * nzc * g.count;
     const SpinorOut<P> vout = SpinorOut<P>::of(v, r0);
     const SpinorIn<P> ain = SpinorIn<P>::of(buf, 0), bin = SpinorIn<P>::of(buf, rows);
     const SpinorOut<P> aout = SpinorOut<P>::of(buf, 0), bout = SpinorOut<P>::of(buf, rows);
@@ -224,8 +301,8 @@ static int gauss_run(qcd_hip_field v, qcd_hip_field gauge, qcd_hip_field buf, in
     for (int it = 0; it < niter; it++) {
       const bool from_a = (it % 2) == 0;
       const SpinorIn<P> &src = from_a ? ain : bin;
-      if (it == niter - 1) k_gauss<P, true><<<grid, kBlock, 0, st>>>(g, src, U, vout, alpha, nrm);
-      else k_gauss<P, false><<<grid, kBlock, 0, st>>>(g, src, U, from_a ? bout : aout, alpha, nrm);
+      if (it == niter - 1) k_gauss<P, true><<<zgrid, kBlock, 0, st>>>(g, src, U, vout, alpha, nrm);
+      else k_gauss<P, false><<<zgrid, kBlock, 0, st>>>(g, src, U, from_a ? bout : aout, alpha, nrm);
       QH_CHECK(hipGetLastError());
     }
   }
@@ -275,7 +352,8 @@ int qcd_hip_gauss_smear(qcd_hip_field v, qcd_hip_field gauge, double alpha, int 
     qcdhip::set_error("gauss_smear: time-slice %d outside 0..%d or negative iteration count", t, v->lat->L[0] - 1);
     return 1;
   }
-  if (niter == 0) return 0;
+  // alpha = 0: every step is psi / (1 + 0) = psi exactly
+  if (niter == 0 || alpha == 0.0) return 0;
   QH_TRY(gauss_smear_field(v, gauge, alpha, t < 0 ? -1 : t, niter, ctx()->stream));
   QH_CHECK(hipStreamSynchronize(ctx()->stream));
   return 0;
