@@ -587,12 +587,16 @@ int solve_batch(qcd_hip_solver_s *S, qcd_hip_field x, qcd_hip_field b, int x0, i
     rel[i] = bnorm[i] > 0.0 ? 1.0 : 0.0;
   }
   int bad = 0;
+  std::vector<int> uploaded = act;   // the list the device holds
   for (int it = 0; it < S->prm.max_outer; it++) {
     act.clear();
     for (int i = 0; i < cnt; i++)
       if (rel[i] > S->prm.tol) act.push_back(i);
     if (act.empty()) break;
-    QH_TRY(upload_ids(S, act, st));
+    if (act != uploaded) {   // most outer steps keep every rhs active: no upload, no host round trip
+      QH_TRY(upload_ids(S, act, st));
+      uploaded = act;
+    }
     const int na = (int)act.size();
     switch (S->prm.inner_prec) {
       case QCD_HIP_F64: QH_TRY(outer_step<QCD_HIP_F64>(S, x, x0, na, kappa, mu, st)); break;
