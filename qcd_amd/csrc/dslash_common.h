@@ -349,10 +349,6 @@ struct DslashArgs {
   unsigned nblk;                   // workgroups per batch row
   unsigned pairs;                  // MAP_ZMARCH2: lanes carry batch rows 2p and 2p+1; pairs = ceil(count / 2)
   unsigned rgroup;                 // z-march batches: rows per group (block_row); 0 = the whole batch
-  // MAP_MMA: order of the 16-column blocks of a plane (mma_tile); bpu = 0: in
-  // index order, else units of bpu blocks (whole x-rows), wu units per x-tile,
-  // upr units per x-line
-  unsigned tile_bpu, tile_wu, tile_upr;
 };
 
 template <int PS>

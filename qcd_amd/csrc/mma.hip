@@ -3,11 +3,12 @@
 // For FP16-storage spinors (per-site norm) and FP16 links, batches of
 // right-hand sides apply each link to a panel of colour vectors with
 // v_mfma_f32_4x4x4_16b_f16: 16 independent 4x4x4 products per instruction,
-// one per site.  A wavefront owns 16 sites x 4 right-hand sides; lane
-// 4 b + n holds site b and rhs n.  For one hop of site b:
-//   A (4 x 4 per block, lane n holds row n): row i of the phased link
+// one per site.  In the instruction's layout lane 4 b + n holds site b and
+// rhs n.  For one hop of site b:
+//   A (4 x 4 per block, lane n holds row n): row n of the phased link
 //     W = e^{+-i theta/L} U (or U^+), as (re, -im) pairs for the real part of
-//     the product and (im, re) pairs for the imaginary part; row 3 is zero;
+//     the product and (im, re) pairs for the imaginary part; row 3 is never
+//     read back;
 //   B (4 x 4 per block, lane n holds column n): the spin-projected and
 //     rebuilt neighbour spinor chi_s = ((1 -+ g_mu) psi)_s of rhs n for one
 //     spin s, colours 0, 1 in the first k step, colour 2 (and a zero) in the
@@ -17,21 +18,23 @@
 // So one hop is 4 spins x {Re, Im} x 2 k steps = 16 MFMAs accumulating into
 // eight 4-float accumulators that hold the site's whole hopping sum: the
 // colour products, the rebuild of the lower spins and the sum over the eight
-// hops all run on the matrix cores.  The vector ALUs only project (six packed
-// FP16 adds per hop), rotate the two lower spins, scale by the site norms and
-// build W.  Each lane's output is its own (site, rhs), so the epilogues and
-// the FP16 store are those of the one-rhs kernels.
+// hops all run on the matrix cores.
+//
+// The loads do not use that layout (four rhs, four separate fields, in every
+// four adjacent lanes): lanes load 16 consecutive sites of one rhs, project
+// and scale there, and hand the six projected half2 of a hop to the MFMA
+// layout by ds_bpermute; the hopping sum comes back the same way for a
+// coalesced epilogue and store.  A wavefront walks 12 z planes of its 16
+// columns with the two previous input planes in registers (mma_march).
 //
 // Per-site norms: the MFMA inputs are FP16, so each neighbour spinor enters
 // scaled by n_nbr / n_ref, with n_ref the largest of the site's eight
 // neighbour norms for that rhs (all factors <= 1, no overflow), and the sum is
 // multiplied by n_ref at the end.
 //
-// The same hop core serves the full-lattice Dslash (D, Q) and the even-odd
-// hops (M^-1 H, M b - H) of the batched inner solve.
+// The same core serves the full-lattice Dslash (D, Q) and the even-odd hops
+// (M^-1 H, M b - H) of the batched inner solve.
 #include "eo_common.h"
-
-#include <numeric>
 
 namespace qcdhip {
 
@@ -307,23 +310,6 @@ __device__ __forceinline__ MmaTile mma_tile(unsigned ncb, unsigned nzc, unsigned
   return t;
 }
 
-// Physical column block of the o-th block of a plane in traversal order.  In
-// index order the y neighbours of a block lie L0 L1 / 16 blocks away, and the
-// wave that reads them as its own columns runs that many places earlier or
-// later: many z steps apart in a z-march, long enough for the plane to leave
-// L2.  The tiled order runs x-tiles of wu units (a unit: the whole x-rows
-// that bpu blocks fill exactly) with y outside the units of a tile, so a
-// block's y neighbours are bpu wu blocks away; x neighbours across a tile
-// edge are the price.
-__device__ __forceinline__ unsigned tile_block(unsigned o, const DslashArgs &a) {
-  if (a.tile_bpu == 0) return o;
-  const unsigned bpu = a.tile_bpu, wu = a.tile_wu;
-  const unsigned j = o % bpu, r = o / bpu;
-  const unsigned ui = r % wu, r2 = r / wu;
-  const unsigned y = r2 % a.L[2], xt = r2 / a.L[2];
-  return j + bpu * (xt * wu + ui + a.tile_upr * y);
-}
-
 // The z-march body shared by the full-lattice and even-odd kernels.  PP:
 // columns per plane (EO: half columns); par: parity of the output sites (EO).
 // epi(acc, centre, s, h, view, store) finishes one site in the load lane:
@@ -341,8 +327,7 @@ __device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, con
   const unsigned lane = threadIdx.x % 64;
   // load role: column lane % 16 of the block, row lane / 16 of the group
   // (columns and rows past the end read the last ones and store nothing)
-  const unsigned blk = tile_block(t.blk, a);
-  const unsigned cl0 = 16 * blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
+  const unsigned cl0 = 16 * t.blk + (lane & 15u), rl = 4 * t.group + (lane >> 4);
   const bool store = cl0 < PP && rl < a.count;
   const unsigned cl = cl0 < PP ? cl0 : PP - 1;
   const RhsView<QCD_HIP_F16> lv = rhs_view(a, in_all, out_all, b_all, rl < a.count ? rl : a.count - 1);
@@ -357,7 +342,7 @@ __device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, con
   }
   const unsigned t0 = st.t;
   // MFMA role: column lane / 4, row lane % 4
-  const unsigned cm0 = 16 * blk + (lane >> 2);
+  const unsigned cm0 = 16 * t.blk + (lane >> 2);
   const unsigned cm = cm0 < PP ? cm0 : PP - 1;
   const int addr = (int)(4u * load_lane_of(lane));                      // MFMA lane <- load lane
   const int back = (int)(4u * (4u * (lane & 15u) + (lane >> 4)));     // load lane <- MFMA lane
@@ -502,44 +487,13 @@ static unsigned mma_zlen(unsigned Lz) {
   return r < 1 || r > Lz ? Lz : r;
 }
 
-// x-rows per x-tile of the block order (tile_block); QCD_HIP_MMA_XTILE
-// overrides the default, 0 = index order
-static unsigned mma_xtile() {
-  static const unsigned r = [] {
-    const char *e = getenv("QCD_HIP_MMA_XTILE");
-    return e ? (unsigned)std::atoi(e) : 4u;
-  }();
-  return r;
-}
-
-// the tiled block order for rows of L0p columns (EO: half columns): units of
-// whole x-rows that 16-column blocks fill exactly, as many units per x-tile
-// as fit in mma_xtile() rows and divide the x-line; index order when the
-// x extent is not a whole number of units
-static void mma_block_order(DslashArgs &a, unsigned L0p) {
-  a.tile_bpu = a.tile_wu = a.tile_upr = 0;
-  const unsigned X = mma_xtile();
-  if (X == 0 || L0p == 0) return;
-  const unsigned g = std::gcd(L0p, 16u), span = L0p / g * 16;   // lcm(L0p, 16) columns
-  const unsigned rows = span / L0p, bpu = span / 16;
-  if (a.L[1] % rows) return;
-  const unsigned upr = a.L[1] / rows;
-  unsigned wu = 1;
-  for (unsigned w = 1; w <= upr; w++)
-    if (upr % w == 0 && w * rows <= X) wu = w;
-  a.tile_bpu = bpu;
-  a.tile_wu = wu;
-  a.tile_upr = upr;
-}
-
-// the z-march fields of a for PP columns per plane (rows of L0p columns);
-// returns the grid: workgroups of kBlock / 64 wavefronts over column blocks x
-// z chunks x rhs groups
-static unsigned mma_setup(DslashArgs &a, unsigned PP, unsigned L0p) {
+// the z-march fields of a for PP columns per plane; returns the grid:
+// workgroups of kBlock / 64 wavefronts over column blocks x z chunks x rhs
+// groups
+static unsigned mma_setup(DslashArgs &a, unsigned PP) {
   a.nblk = (PP + 15) / 16;
   a.zlen = mma_zlen(a.L[3]);
   a.rgroup = mma_rgroup();
-  mma_block_order(a, L0p);
   const unsigned nzc = (a.L[3] + a.zlen - 1) / a.zlen;
   const unsigned waves = a.nblk * nzc * ((a.count + 3) / 4), per = kBlock / 64;
   return (waves + per - 1) / per;
@@ -551,7 +505,7 @@ int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_
     set_error("dslash: the matrix-core traversal needs FP16 spinors and links and the D or Q epilogue");
     return 1;
   }
-  const unsigned grid = mma_setup(a, a.L[0] * a.L[1] * a.L[2], a.L[0]);
+  const unsigned grid = mma_setup(a, a.L[0] * a.L[1] * a.L[2]);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
   const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);
@@ -574,7 +528,7 @@ int eo_mma_launch(int epi, const EoArgs &e0, qcd_hip_field out, int ro, qcd_hip_
     set_error("eo: the matrix-core hop needs an even time extent");
     return 1;
   }
-  const unsigned grid = mma_setup(e.d, e.d.L[0] * e.d.L[1] * e.d.L[2] / 2, e.d.L[0] / 2);
+  const unsigned grid = mma_setup(e.d, e.d.L[0] * e.d.L[1] * e.d.L[2] / 2);
   const SpinorIn<QCD_HIP_F16> si = SpinorIn<QCD_HIP_F16>::of(in, ri);
   const SpinorIn<QCD_HIP_F16> sb = b ? SpinorIn<QCD_HIP_F16>::of(b, rb) : si;
   const SpinorOut<QCD_HIP_F16> so = SpinorOut<QCD_HIP_F16>::of(out, ro);
