@@ -130,3 +130,34 @@ def test_mma_even_odd_schur(gpu, nrhs):
     rev = schur(vs[::-1][:17])
     for k, y in enumerate(rev):
         assert np.array_equal(y, ys[nrhs - 1 - k]), k
+
+
+def test_mma_z_chunks_of_unequal_length(gpu):
+    """The matrix-core kernels walk z in chunks of 12 planes: z extents of 13
+    (full lattice, D over 5 rhs) and 14 (even-odd Schur over 16 rhs) leave a
+    last chunk of 1 and 2 planes.  Every rhs against the oracle."""
+    Q = gpu
+    m, tm = 1 / (2 * 0.13) - 4, 0.03
+    L = (8, 4, 4, 13)
+    lat, u, g, srcs, x = _fields(Q, L, 5, 830, (np.pi, 0.3, 0.0, -0.4))
+    y = Q.Spinor(lat, Q.F16, nrhs=5)
+    Q.dslash_set_config(x, g, MMA, 0)
+    Q.dslash(y, x, g, m, tm, Q.OP_D)
+    geo = O.Geometry(L, (np.pi, 0.3, 0.0, -0.4))
+    for r in range(5):
+        assert rel_l2(y.download(r), O.dslash(geo, srcs[r], u, m, tm)) <= 4e-3, r
+    L, theta = (8, 4, 4, 14), (np.pi, 0.0, 0.2, 0.0)
+    u = Q.gauge_generate(L, 0.3, 840)
+    lat = Q.Lattice(L, theta)
+    half = Q.Lattice((L[0] // 2,) + tuple(L[1:]))
+    g64 = Q.Gauge(lat).upload(u)
+    g = Q.Gauge(lat, Q.F16)
+    Q.convert(g, g64)
+    vs = [Q.eo_split(Q.spinor_gaussian(L, 850 + r), L)[1] for r in range(16)]
+    xb = Q.Spinor(half, Q.F16, nrhs=16)
+    for r, v in enumerate(vs):
+        xb.upload(v, rhs=r)
+    yb = Q.Spinor(half, Q.F16, nrhs=16)
+    Q.dslash_schur(yb, xb, g, m, tm)
+    for r, v in enumerate(vs):
+        assert rel_l2(yb.download(r), schur_reference(Q, L, u, v, m, tm, theta)) <= 4e-3, r

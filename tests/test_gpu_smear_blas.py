@@ -156,3 +156,24 @@ def test_dropin_blas(gpu):
         assert np.abs(di.view(vb, 12) - (1j * b).reshape(-1)).max() == 0.0
     finally:
         di.close()
+
+
+@pytest.mark.parametrize("Ls", [(4, 6, 4, 7), (6, 4, 6, 5), (4, 4, 4, 9)])
+def test_gauss_z_chunks_of_unequal_length(gpu, Ls):
+    """The smearing kernel walks z in chunks of ceil(Z / 4) (at least 2)
+    planes; z extents of 7, 5 and 9 leave a shorter last chunk (and Z = 5 a
+    chunk of one plane).  One slice and all slices against the oracle
+    (lib/src/qcd_smearing.c:25-199)."""
+    Q = gpu
+    u = Q.gauge_generate(Ls, 0.25, 801)
+    v = Q.spinor_gaussian(Ls, 802)
+    lat = Q.Lattice(Ls)
+    g = Q.Gauge(lat).upload(u)
+    geo = O.Geometry(Ls)
+    for t in (1, -1):
+        f = Q.Spinor(lat).upload(v)
+        Q.gauss_smear(f, g, 2.5, t, 3)
+        ref = v.copy()
+        for _ in range(3):
+            ref = O.gauss3d(geo, ref, u, 2.5, t) if t >= 0 else O.gauss3d_all(geo, ref, u, 2.5)
+        assert rel_l2(f.download(), ref) <= 1e-13, (Ls, t)
