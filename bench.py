@@ -268,11 +268,15 @@ def source_sites_24c48(L=(48, 24, 24, 24), seed=3003):
     return [tuple(int(rng.integers(0, L[d])) for d in range(4)) for _ in range(4)]
 
 
-def pipeline_48c96_leg(Q, batch=4, seed=5005, keep=None):
+def pipeline_48c96_leg(Q, batches=None, seed=5005, keep=None):
     """48^3x96, eps 0.3, kappa 0.132: 10 APE steps (alpha 0.5) of the links, 12
     point sources at the origin Gauss-smeared (alpha 4.0, 50 iterations, FP64)
     with the APE links, then a 12-rhs mixed solve (FP16 inner, even-odd) with
-    the original links to ||b - D x|| / ||b|| <= 1e-10, checked in FP64."""
+    the original links to ||b - D x|| / ||b|| <= 1e-10, checked in FP64.
+    batches maps the restart length to the rhs per batch: GCR(12) solves the
+    12 rhs as one block (about 210 GB in use); GCR(32) in blocks of 6, as 12
+    would need about 336 GB."""
+    batches = batches or {32: 6, 12: 12}
     L, kappa = (96, 48, 48, 48), 0.132
     lat = Q.Lattice(L)
     g = Q.Gauge(lat)
@@ -299,7 +303,7 @@ def pipeline_48c96_leg(Q, batch=4, seed=5005, keep=None):
     del b1, x1
     solves = {}
     for n in (32, 12):   # invert.ini:8's N_restart, and the shorter cycle
-        S = Q.Solver(lat, g, n_restart=n, tol=1e-10, inner_prec="f16", batch=batch, max_outer=20000, eo=1)
+        S = Q.Solver(lat, g, n_restart=n, tol=1e-10, inner_prec="f16", batch=batches[n], max_outer=20000, eo=1)
         Q.synchronize()
         t0 = time.perf_counter()
         st = S.solve(x, b, kappa)
@@ -308,14 +312,14 @@ def pipeline_48c96_leg(Q, batch=4, seed=5005, keep=None):
         S.close()
         worst = max(Q.b_minus_Dx(x, b, g, kappa, rx=r, rb=r) for r in range(12))
         assert worst <= 1e-10, f"48^3x96 solve not converged: {worst}"
-        solves[n] = {"solve_seconds": t_solve, "solves_per_s": 12 / t_solve, "gcr_iterations": st["inner_iters"],
-                     "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
+        solves[n] = {"solve_seconds": t_solve, "solves_per_s": 12 / t_solve, "batch": batches[n],
+                     "gcr_iterations": st["inner_iters"], "outer_cycles": st["outer"], "worst_b_minus_Dx": worst}
     if keep is not None:   # rhs 0 (a smeared source) for the oracle re-check
         keep.update({"u": g.download(), "L": L, "kappa": kappa, "x0": x.download(0), "b0": b.download(0),
                      "out": solves[12]})
     out = {"config": "48^3x96, eps 0.3, kappa 0.132, 10 APE steps alpha 0.5, 12 point sources at the origin "
                      "Gauss-smeared (alpha 4.0, 50 iterations, FP64), mixed FP16-inner even-odd solve, tol 1e-10",
-           "ape_seconds": t_ape, "smear_seconds": t_smear, "batch": batch, **solves[32],
+           "ape_seconds": t_ape, "smear_seconds": t_smear, **solves[32],
            "pipeline_seconds": t_ape + t_smear + solves[32]["solve_seconds"], "gcr12": solves[12]}
     return out
 
@@ -507,7 +511,7 @@ def main():
             k.pop("x")
         if not args.no_pipeline48:
             kept["pipeline_48c96_gcr12"] = {}
-            solves["pipeline_48c96"] = pipeline_48c96_leg(Q, batch=6, keep=kept["pipeline_48c96_gcr12"])
+            solves["pipeline_48c96"] = pipeline_48c96_leg(Q, keep=kept["pipeline_48c96_gcr12"])
             c5 = solves["pipeline_48c96"]
             log(f"48^3x96 pipeline: APE {c5['ape_seconds']:.2f}s, smearing {c5['smear_seconds']:.2f}s, "
                 f"solve {c5['solve_seconds']:.2f}s ({c5['solves_per_s']:.2f} solves/s); GCR(12) solve "
