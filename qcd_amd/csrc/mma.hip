@@ -68,14 +68,26 @@ struct RhsF16 {
 // field or the phases change): for direction mu and hop kind k of site x
 // (0: forward, W = e^{+i theta_mu/L_mu} U_mu(x); 1: backward,
 // W = e^{-i theta_mu/L_mu} U_mu(x - mu)^+), row i of W as three
-// (Re W_ic, -Im W_ic) FP16 pairs, 12 bytes, at byte (x * 3 + i) * 12 of array
-// 2 mu + k.  Both kinds sit at the site that applies them, so a site's eight
-// link rows need only its own index.  The Im operand (Im W_ic, Re W_ic) is one
-// packed multiply away.  288 bytes per site.
+// (Re W_ic, -Im W_ic) FP16 pairs, 12 bytes, at byte (hop_record(x) * 3 + i) * 12
+// of array 2 mu + k.  Both kinds sit at the site that applies them, so a
+// site's eight link rows need only its own index.  The Im operand
+// (Im W_ic, Re W_ic) is one packed multiply away.  288 bytes per site.
+//
+// Records go parity-major when the time extent is even: the sites of parity p
+// at records p V/2 + (x >> 1), the even-odd half-lattice order.  An even-odd
+// hop (all output sites of one parity) then reads one contiguous run of
+// records per block of half columns; in site order it would read every other
+// record and fetch the lines of both parities.  The full-lattice hop reads
+// two runs, one per parity.
 struct HopLinks {
   const char *p[8];   // array 2 mu + k
 };
 constexpr size_t kHopLinkBytes = 8 * 36;   // per site
+
+// record of site s = t + L0 (x + L1 (y + L2 z)) of parity par
+__device__ __forceinline__ unsigned hop_record(unsigned s, unsigned par, const DslashArgs &a) {
+  return (a.L[0] & 1u) ? s : par * (a.V / 2) + (s >> 1);
+}
 
 template <int PG>
 __global__ void __launch_bounds__(kBlock) k_hop_links(DslashArgs a, GaugeIn<PG> U, HopLinks out) {
@@ -109,7 +121,8 @@ __global__ void __launch_bounds__(kBlock) k_hop_links(DslashArgs a, GaugeIn<PG> 
         const float wr = fmaf(x.re, pr, -(x.im * qi)), wi = fmaf(x.re, qi, x.im * pr);
         w[3 * i + c] = as_u32(__floats2half2_rn(wr, -wi));
       }
-    U32x3 *o = (U32x3 *)(const_cast<char *>(out.p[2 * mu + kind]) + (size_t)s * 36);
+    U32x3 *o = (U32x3 *)(const_cast<char *>(out.p[2 * mu + kind]) +
+                         (size_t)hop_record(s, (st.t + st.x + st.y + st.z) & 1u, a) * 36);
 #pragma unroll
     for (int i = 0; i < 3; i++) o[i] = U32x3{w[3 * i], w[3 * i + 1], w[3 * i + 2]};
   }
@@ -346,8 +359,15 @@ __device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, con
   const unsigned cm = cm0 < PP ? cm0 : PP - 1;
   const int addr = (int)(4u * load_lane_of(lane));                      // MFMA lane <- load lane
   const int back = (int)(4u * (4u * (lane & 15u) + (lane >> 4)));     // load lane <- MFMA lane
-  const int bsite = (int)(4u * (lane >> 2));                            // load lane of row 0, same column
   const unsigned ii = (lane & 3u) < 3u ? (lane & 3u) : 2u;
+  // hop-link record of the MFMA lane's site at plane z: even-odd, the output
+  // parity's run (par V/2 + half-lattice index); full lattice, the parity of
+  // (t, x, y) of column cm flips with z
+  unsigned pcm = 0;
+  if constexpr (!EO) {
+    const unsigned r = cm / a.L[0], y = r / a.L[1];
+    pcm = (cm - r * a.L[0] + (r - y * a.L[1]) + y) & 1u;
+  }
   const unsigned z0 = t.zc * a.zlen, z1 = min(z0 + a.zlen, Lz);
   Plane pm, pc;   // input planes z - 1 and z of the load lane's column
   {
@@ -367,7 +387,7 @@ __device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, con
     } else {
       st.s = cl + P3 * z;
     }
-    const unsigned sm = EO ? pull(st.s, bsite) : cm + P3 * z;
+    const unsigned rec = EO ? par * (a.V / 2) + cm + PP * z : hop_record(cm + P3 * z, (pcm + z) & 1u, a);
     const unsigned ip = cl + PP * (z + 1 == Lz ? 0 : z + 1);
     const unsigned nbr[6] = {neighbour<0, +1>(st, a) >> sh, neighbour<0, -1>(st, a) >> sh,
                              neighbour<1, +1>(st, a) >> sh, neighbour<1, -1>(st, a) >> sh,
@@ -385,7 +405,7 @@ __device__ __forceinline__ void mma_march(const DslashArgs &a, unsigned par, con
     for (int k = 0; k < 6; k++) in.load(nbr[k], r[k]);
     in.load(ip, r[6]);
 #pragma unroll
-    for (int k = 0; k < 8; k++) w[k] = *(const U32x3 *)(hl.p[k] + (sm * 36u + ii * 12u));   // below 2^32 for V <= 2^24
+    for (int k = 0; k < 8; k++) w[k] = *(const U32x3 *)(hl.p[k] + (rec * 36u + ii * 12u));   // below 2^32 for V <= 2^24
     __builtin_amdgcn_sched_barrier(0);   // keep the scheduler from sinking the loads
     float nref = pm.n;
 #pragma unroll
