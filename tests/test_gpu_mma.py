@@ -31,13 +31,15 @@ def _fields(Q, L, nrhs, seed, theta=(np.pi, 0.0, 0.0, 0.0)):
 
 
 @pytest.mark.parametrize("L,theta", [((16, 8, 8, 8), (np.pi, 0.0, 0.0, 0.0)),
-                                     ((6, 2, 2, 5), (np.pi, 0.4, -0.3, 1.1))])
+                                     ((6, 2, 2, 5), (np.pi, 0.4, -0.3, 1.1)),
+                                     ((5, 4, 2, 6), (np.pi, 0.2, 0.0, -0.7))])
 @pytest.mark.parametrize("nrhs", [1, 3, 5, 16])
 def test_mma_dslash_per_rhs(gpu, L, theta, nrhs):
     """D and Q with twisted mass over 1, 3, 5 and 16 rhs (ragged groups of 4),
-    on a lattice whose volume (120) is not a multiple of the 16-site block and
-    with phases in every direction; each rhs against the oracle and against the
-    vector-ALU two-rhs kernel."""
+    on lattices whose volumes (120, 240) are not multiples of the 16-site block,
+    with phases in every direction and an odd time extent (hop links in site
+    order rather than parity-major); each rhs against the oracle and against
+    the vector-ALU two-rhs kernel."""
     Q = gpu
     m, tm = 1 / (2 * 0.13) - 4, 0.02
     lat, u, g, srcs, x = _fields(Q, L, nrhs, 700 + nrhs, theta)
