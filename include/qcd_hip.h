@@ -234,6 +234,13 @@ int qcd_hip_plaquette(qcd_hip_field gauge64, double *plaq);
  * field.  No reference counterpart (the reference has no even-odd form). */
 int qcd_hip_dslash_schur(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, double m, double tm);
 
+/* z steps per lane of the even-odd hops' z-marching kernel, for every later
+ * even-odd hop of the process: 0 = chosen per launch (the default, or the
+ * value of QCD_HIP_EO_ZLEN), 1 = one output site per lane, n > 1 = z chunks
+ * of n planes.  Every choice gives bit-identical results.  No reference
+ * counterpart. */
+int qcd_hip_eo_set_zlen(int zlen);
+
 /* ------------------------------------------------------------ serving
  * Continuous serving of propagator-solve requests (no reference
  * counterpart; each request is the invert.c solve of its own configuration).

@@ -122,6 +122,7 @@ def lib():
         "qcd_hip_spinor_gaussian_host": (i, [_dp, lg, ctypes.c_uint64]),
         "qcd_hip_dslash": (i, [_vp, _vp, _vp, d, d, i]),
         "qcd_hip_dslash_schur": (i, [_vp, _vp, _vp, d, d]),
+        "qcd_hip_eo_set_zlen": (i, [i]),
         "qcd_hip_ape_smear": (i, [_vp, _vp, d, i]),
         "qcd_hip_stout_smear": (i, [_vp, _vp, d, i, i]),
         "qcd_hip_plaquette": (i, [_vp, _dp]),
@@ -358,6 +359,11 @@ def point_source(L, site, spin, col):
 # ------------------------------------------------------------ operators / BLAS
 def dslash(out, inp, gauge, m, tm=0.0, op=OP_D):
     _check(lib().qcd_hip_dslash(out.h, inp.h, gauge.h, float(m), float(tm), int(op)), "qcd_hip_dslash")
+
+
+def eo_set_zlen(zlen):
+    """z steps per lane of the even-odd hops (0 = automatic, 1 = one site per lane)."""
+    _check(lib().qcd_hip_eo_set_zlen(int(zlen)), "qcd_hip_eo_set_zlen")
 
 
 def dslash_schur(out, inp, gauge, m, tm=0.0):

@@ -11,6 +11,9 @@
 // The hopping sum uses the projector/link/phase arithmetic of the full Dslash.
 #include "eo_common.h"
 
+#include <atomic>
+#include <type_traits>
+
 namespace qcdhip {
 
 // both hops of direction MU into acc, the neighbour spinors read from the
@@ -148,6 +151,159 @@ __global__ void __launch_bounds__(kBlock) k_eo_hop(EoArgs e, SpinorIn<PS> in_all
       for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     }
     rv.out.template store<true>(h, o);
+  }
+  if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
+}
+
+// acc of one or two rows pinned in registers and a scheduling fence: the
+// loads of the next hop direction are not hoisted above this point (every
+// precision: the z-march carries a ring of hops on top of the accumulators)
+template <typename R>
+__device__ __forceinline__ void zm_order_point(Cx<R> acc[4][3]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) asm volatile("" : "+v"(acc[q][c].re), "+v"(acc[q][c].im));
+  __builtin_amdgcn_sched_barrier(0);
+}
+template <typename R>
+__device__ __forceinline__ void zm_order_point(Cx<R> acc0[4][3], Cx<R> acc1[4][3]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      asm volatile("" : "+v"(acc0[q][c].re), "+v"(acc0[q][c].im));
+      asm volatile("" : "+v"(acc1[q][c].re), "+v"(acc1[q][c].im));
+    }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// z-marching EO hop: a lane owns one half column j (the t-pair 2j', 2j'+1 of
+// one (x, y)) of NR batch rows and steps through planes z0 .. z1-1.  Its output
+// site at plane z has t = 2j' + odd(z), odd(z) = (par + x + y + z) & 1, and
+// half-lattice entry j + PP z.  The z neighbours of that site are the input
+// half column j at planes z -+ 1, and the forward one, (t, x, y, z + 1), is also
+// the backward z neighbour of the lane's output site at z + 2.  So when the lane
+// loads it for the forward hop it also forms the backward hop of step z + 2,
+// U_z(t, x, y, z + 1)^+ (1 + g_z) psi, and carries it two steps (a ring of two
+// colour-multiplied half spinors per row).  Every input spinor of a z chunk is
+// then read once by its own lane for both z hops, and the t, x, y neighbours
+// at plane z are read by the lanes of nearby columns at the same step.
+// Each hop is computed from the same values with the same arithmetic as
+// k_eo_hop / k_eo_hop2 and added in the same order (t, x, y, z; forward, then
+// backward), so the results are bit-identical.  NR = 2: rows 2p and 2p + 1 of
+// pair p, each link loaded once for both (an odd batch's last pair repeats
+// its row and stores it once).
+template <int PS, int PG, int EPI, bool RED, int NR>
+__global__ void __launch_bounds__(kBlock) k_eo_hop_zm(EoArgs e, SpinorIn<PS> in_all, GaugeIn<PG> U,
+                                                      SpinorOut<PS> out_all, SpinorIn<PS> b_all,
+                                                      double *__restrict__ partials) {
+  static_assert(NR == 1 || NR == 2, "one or two rows per lane");
+  using R = typename Prec<PS>::R;
+  const DslashArgs &a = e.d;
+  const unsigned nrow = NR == 2 ? (a.count + 1) / 2 : a.count;
+  const BlockRow br = eo_row_of(nrow, NR == 2 ? max(1u, e.rgroup / 2) : e.rgroup, a.nblk);
+  const unsigned r0 = NR * br.row, r1 = NR == 2 ? min(r0 + 1, a.count - 1) : r0;
+  RhsView<PS> v[NR] = {rhs_view(a, in_all, out_all, b_all, r0)};
+  if constexpr (NR == 2) v[1] = rhs_view(a, in_all, out_all, b_all, r1);
+  const unsigned cb = br.blk % a.col_blocks, zc = br.blk / a.col_blocks;
+  const unsigned P3 = a.L[0] * a.L[1] * a.L[2], PP = P3 / 2, Lz = a.L[3];
+  const unsigned j = cb * kBlock + threadIdx.x;
+  const unsigned z0 = zc * a.zlen, z1 = min(z0 + a.zlen, Lz);
+  double red = 0.0;
+  if (j < PP && z0 < Lz) {
+    Site st;
+    st.valid = true;
+    const unsigned l0 = 2 * j;
+    const unsigned rr = l0 / a.L[0];
+    const unsigned t0 = l0 - rr * a.L[0];
+    st.y = rr / a.L[1];
+    st.x = rr - st.y * a.L[1];
+    const Cx<R> php{(R)a.ph_re[3], (R)a.ph_im[3]};
+    const Cx<R> phm{(R)a.ph_re[3], -(R)a.ph_im[3]};
+    // cz[n][q]: backward z hop (two half spinors) of row n's output site at a
+    // plane z with (z - z0) % 2 == q; a step uses its slot and refills it with
+    // the hop of z + 2, so the slot index is fixed per unrolled step
+    Cx<R> cz[NR][2][2][3];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const unsigned zq = z0 + q;   // output plane; its backward neighbour is at zq - 1
+      const unsigned odd = (e.par + st.x + st.y + zq) & 1u;
+      const unsigned sb = l0 + odd + P3 * (zq == 0 ? Lz - 1 : zq - 1);   // zq <= Lz: in range
+      typename Prec<PG>::T u[9];
+      U.load_raw(3, sb, u);
+#pragma unroll
+      for (int n = 0; n < NR; n++) {
+        typename Prec<PS>::T r[12];
+        float nn = 0.f;
+        v[n].in.load_raw(sb >> 1, r, nn);
+        hop_half_raw<3, +1, true, PS, PG>(cz[n][q][0], cz[n][q][1], r, nn, u, phm);
+      }
+    }
+    auto step = [&](const unsigned z, auto slot) {
+      constexpr int S = decltype(slot)::value;
+      const unsigned odd = (e.par + st.x + st.y + z) & 1u;
+      st.t = t0 + odd;
+      st.z = z;
+      st.s = l0 + odd + P3 * z;
+      const unsigned h = j + PP * z;
+      Cx<R> acc[NR][4][3];
+#pragma unroll
+      for (int n = 0; n < NR; n++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[n][q][c] = Cx<R>{R(0), R(0)};
+      if constexpr (NR == 1) {
+        eo_hop_dir<0, PS, PG, R>(acc[0], a, st, v[0].in, U);
+        zm_order_point(acc[0]);
+        eo_hop_dir<1, PS, PG, R>(acc[0], a, st, v[0].in, U);
+        zm_order_point(acc[0]);
+        eo_hop_dir<2, PS, PG, R>(acc[0], a, st, v[0].in, U);
+        zm_order_point(acc[0]);
+      } else {
+        eo_hop_dir2<0, PS, PG, R>(acc[0], acc[1], a, st, v[0].in, v[1].in, U);
+        zm_order_point(acc[0], acc[1]);
+        eo_hop_dir2<1, PS, PG, R>(acc[0], acc[1], a, st, v[0].in, v[1].in, U);
+        zm_order_point(acc[0], acc[1]);
+        eo_hop_dir2<2, PS, PG, R>(acc[0], acc[1], a, st, v[0].in, v[1].in, U);
+        zm_order_point(acc[0], acc[1]);
+      }
+      // z: forward hop from (t, x, y, z + 1), whose backward hop for the output
+      // site at z + 2 is formed from the same stored values; then the carried
+      // backward hop of this step
+      const bool ahead = z + 2 < z1;   // uniform over the workgroup
+      const unsigned sf = z + 1 == Lz ? l0 + odd : st.s + P3;
+      typename Prec<PG>::T u[9], u2[9];
+      U.load_raw(3, st.s, u);
+      if (ahead) U.load_raw(3, sf, u2);
+#pragma unroll
+      for (int n = 0; n < NR; n++) {
+        typename Prec<PS>::T r[12];
+        float nn = 0.f;
+        v[n].in.load_raw(sf >> 1, r, nn);
+        Cx<R> g0[3], g1[3];
+        hop_half_raw<3, -1, false, PS, PG>(g0, g1, r, nn, u, php);
+        rebuild_add<3, -1>(acc[n], g0, g1);
+        rebuild_add<3, +1>(acc[n], cz[n][S][0], cz[n][S][1]);
+        if (ahead) hop_half_raw<3, +1, true, PS, PG>(cz[n][S][0], cz[n][S][1], r, nn, u2, phm);
+      }
+#pragma unroll
+      for (int n = 0; n < NR; n++) {
+        if (n == 1 && r1 == r0) break;
+        Cx<R> o[12];
+        eo_epilogue<EPI>(o, acc[n], e, v[n].b, h);
+        if (RED) {
+#pragma unroll
+          for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+        }
+        v[n].out.template store<true>(h, o);
+      }
+    };
+    for (unsigned z = z0; z < z1; z += 2) {
+      step(z, std::integral_constant<int, 0>{});
+      if (z + 1 < z1) step(z + 1, std::integral_constant<int, 1>{});
+    }
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
 }
@@ -363,9 +519,71 @@ static int eo_mma_min_rhs() {
   return n;
 }
 
+// z steps per lane of the z-marching hop (k_eo_hop_zm); 1 runs the one-site
+// kernels, 0 lets eo_zlen choose.  QCD_HIP_EO_ZLEN sets the start value,
+// qcd_hip_eo_set_zlen changes it.
+static std::atomic<int> &eo_zlen_setting() {
+  static std::atomic<int> v([] {
+    const char *s = getenv("QCD_HIP_EO_ZLEN");
+    return s ? std::max(0, std::atoi(s)) : 0;
+  }());
+  return v;
+}
+
+// The automatic choice follows Schur-operator timings (one lane per half
+// column, rows = rhs per launch or rhs pairs): the z-march gains once the
+// launch has about 2^17 (row, half-site) items.  24^3x48: one FP64 rhs -24% at
+// 6 steps, one FP32 rhs -18% at 8, 12 FP32 / FP16 rhs -12% / -3.5% at 8;
+// 48^3x96: one FP64 rhs -24% and 12 FP16 rhs -15% at 4-12.  With fewer items
+// the shorter grid loses (16^3x32, one FP64 rhs: +6% at 2 steps), and short z
+// extents gain only at 2 steps (16^3x32, 12 FP16 rhs: -2%).
+static unsigned eo_zlen(const EoArgs &e, unsigned rows) {
+  const int set = eo_zlen_setting().load();
+  const unsigned Lz = e.d.L[3];
+  unsigned zc = 1;
+  if (set > 0) {
+    zc = (unsigned)set;
+  } else {
+    const double items = (double)rows * e.Vh;
+    if (items < 131072.0) zc = 1;
+    else if (Lz <= 16) zc = 2;
+    else zc = (unsigned)std::min(8.0, std::max(2.0, std::floor(items / 65536.0)));
+  }
+  const unsigned z = std::min(zc, Lz);
+  // workgroups per row must not exceed the one-site kernels' (the reduction
+  // buffers are sized for those)
+  const unsigned PP = e.d.L[0] * e.d.L[1] * e.d.L[2] / 2;
+  if ((size_t)((PP + kBlock - 1) / kBlock) * ((Lz + z - 1) / z) > (size_t)grid_for(e.Vh)) return 1;
+  return z;
+}
+
+template <int PS, int PG, int NR>
+static int eo_zm_launch(int epi, bool red, EoArgs e, unsigned zlen, unsigned rows, const SpinorIn<PS> &si,
+                        const GaugeIn<PG> &gu, const SpinorOut<PS> &so, const SpinorIn<PS> &sb, double *partials,
+                        hipStream_t st) {
+  const unsigned PP = e.d.L[0] * e.d.L[1] * e.d.L[2] / 2;
+  e.d.col_blocks = (PP + kBlock - 1) / kBlock;
+  e.d.zlen = zlen;
+  e.d.nblk = e.d.col_blocks * ((e.d.L[3] + zlen - 1) / zlen);
+  const dim3 gr(e.d.nblk * rows);
+#define QH_Z(E, RD) k_eo_hop_zm<PS, PG, E, RD, NR><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
+  switch (epi * 2 + (red ? 1 : 0)) {
+    case 0: QH_Z(EO_MINV, false); break;
+    case 2: QH_Z(EO_SCHUR, false); break;
+    case 4: QH_Z(EO_SUB, false); break;
+    case 5: QH_Z(EO_SUB, true); break;
+    case 6: QH_Z(EO_SUB_MINV, false); break;
+    default: set_error("eo: unsupported epilogue %d (reduction %d)", epi, (int)red); return 1;
+  }
+#undef QH_Z
+  QH_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <int PS, int PG>
 static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
-                        int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st) {
+                        int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st,
+                        int *grid_out) {
   const SpinorIn<PS> si = SpinorIn<PS>::of(in, ri);
   const SpinorIn<PS> sb = b ? SpinorIn<PS>::of(b, rb) : si;
   const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
@@ -374,6 +592,20 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
   if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16) {
     if ((int)e.d.count >= eo_mma_min_rhs() && !red && (epi == EO_MINV || epi == EO_SCHUR))
       return eo_mma_launch(epi, e, out, ro, in, ri, g, b, rb, st);
+  }
+  {
+    const bool two = PS != QCD_HIP_F64 && e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) &&
+                     eo_two_rhs();
+    const unsigned rows = two ? (e.d.count + 1) / 2 : e.d.count;
+    const unsigned zlen = eo_zlen(e, rows);
+    if (zlen > 1) {
+      if (red && grid_out) *grid_out = (int)(((e.d.L[0] * e.d.L[1] * e.d.L[2] / 2 + kBlock - 1) / kBlock) *
+                                             ((e.d.L[3] + zlen - 1) / zlen));
+      if constexpr (PS != QCD_HIP_F64) {
+        if (two) return eo_zm_launch<PS, PG, 2>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
+      }
+      return eo_zm_launch<PS, PG, 1>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
+    }
   }
   if constexpr (PS != QCD_HIP_F64) {
     if (e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) && eo_two_rhs()) {
@@ -411,17 +643,17 @@ int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_
   e.d.out_rs = (unsigned)out_rs;
   e.d.b_rs = (unsigned)b_rs;
   const int grid = grid_for(e.Vh);
-  if (grid_out) *grid_out = grid;
+  if (grid_out) *grid_out = grid;   // the z-marching hop sets its own count
   e.d.nblk = (unsigned)grid;
   e.rgroup = eo_row_group(in, count);
   const dim3 gr((unsigned)grid * (unsigned)count);
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
-    return eo_launch_pp<QCD_HIP_F64, QCD_HIP_F64>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+    return eo_launch_pp<QCD_HIP_F64, QCD_HIP_F64>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st, grid_out);
   if (ps == QCD_HIP_F32 && pg == QCD_HIP_F32)
-    return eo_launch_pp<QCD_HIP_F32, QCD_HIP_F32>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+    return eo_launch_pp<QCD_HIP_F32, QCD_HIP_F32>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st, grid_out);
   if (ps == QCD_HIP_F16 && pg == QCD_HIP_F16)
-    return eo_launch_pp<QCD_HIP_F16, QCD_HIP_F16>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st);
+    return eo_launch_pp<QCD_HIP_F16, QCD_HIP_F16>(epi, red, e, gr, out, ro, in, ri, g, b, rb, partials, st, grid_out);
   set_error("eo: unsupported precision pair (spinor %d, gauge %d)", ps, pg);
   return 1;
 }
@@ -529,6 +761,15 @@ int eo_accumulate_p(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od,
 using namespace qcdhip;
 
 extern "C" {
+
+int qcd_hip_eo_set_zlen(int zlen) {
+  if (zlen < 0) {
+    set_error("eo_set_zlen: zlen must be >= 0");
+    return 1;
+  }
+  eo_zlen_setting().store(zlen);
+  return 0;
+}
 
 int qcd_hip_dslash_schur(qcd_hip_field out, qcd_hip_field in, qcd_hip_field gauge, double m, double tm) {
   QH_TRY(ensure_init());
