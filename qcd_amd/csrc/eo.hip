@@ -12,6 +12,9 @@
 #include "eo_common.h"
 
 #include <atomic>
+#include <cstdio>
+#include <tuple>
+#include <map>
 #include <type_traits>
 
 namespace qcdhip {
@@ -530,31 +533,36 @@ static std::atomic<int> &eo_zlen_setting() {
   return v;
 }
 
-// The automatic choice follows Schur-operator timings (one lane per half
-// column, rows = rhs per launch or rhs pairs): the z-march gains once the
-// launch has about 2^17 (row, half-site) items.  24^3x48: one FP64 rhs -24% at
-// 6 steps, one FP32 rhs -18% at 8, 12 FP32 / FP16 rhs -12% / -3.5% at 8;
-// 48^3x96: one FP64 rhs -24% and 12 FP16 rhs -15% at 4-12.  With fewer items
-// the shorter grid loses (16^3x32, one FP64 rhs: +6% at 2 steps), and short z
-// extents gain only at 2 steps (16^3x32, 12 FP16 rhs: -2%).
+// z for a chunk of zc planes, or 1 when the z-march grid would have more
+// workgroups per row than the one-site kernels (the reduction buffers are
+// sized for those)
+static unsigned eo_zlen_fits(const EoArgs &e, unsigned zc) {
+  const unsigned Lz = e.d.L[3];
+  const unsigned z = std::min(zc, Lz);
+  const unsigned PP = e.d.L[0] * e.d.L[1] * e.d.L[2] / 2;
+  if ((size_t)((PP + kBlock - 1) / kBlock) * ((Lz + z - 1) / z) > (size_t)grid_for(e.Vh)) return 1;
+  return z;
+}
+
+// The z chunk when it cannot be timed (under graph capture), or as set.
+// From Schur-operator timings: the z-march gains once a launch has about
+// 2^17 (row, half-site) items (24^3x48: one FP64 rhs -24% at 6 steps, one FP32
+// rhs -18% at 8, 12 FP32 / FP16 rhs -12% / -3.5% at 8; 48^3x96: one FP64 rhs
+// -24% and 12 FP16 rhs -15% at 4-12); with fewer the shorter grid loses
+// (16^3x32, one FP64 rhs: +6% at 2 steps); short z extents gain only at 2
+// steps (16^3x32, 12 FP16 rhs: -2%).
 static unsigned eo_zlen(const EoArgs &e, unsigned rows) {
   const int set = eo_zlen_setting().load();
-  const unsigned Lz = e.d.L[3];
   unsigned zc = 1;
   if (set > 0) {
     zc = (unsigned)set;
   } else {
     const double items = (double)rows * e.Vh;
     if (items < 131072.0) zc = 1;
-    else if (Lz <= 16) zc = 2;
+    else if (e.d.L[3] <= 16) zc = 2;
     else zc = (unsigned)std::min(8.0, std::max(2.0, std::floor(items / 65536.0)));
   }
-  const unsigned z = std::min(zc, Lz);
-  // workgroups per row must not exceed the one-site kernels' (the reduction
-  // buffers are sized for those)
-  const unsigned PP = e.d.L[0] * e.d.L[1] * e.d.L[2] / 2;
-  if ((size_t)((PP + kBlock - 1) / kBlock) * ((Lz + z - 1) / z) > (size_t)grid_for(e.Vh)) return 1;
-  return z;
+  return eo_zlen_fits(e, zc);
 }
 
 template <int PS, int PG, int NR>
@@ -580,42 +588,28 @@ static int eo_zm_launch(int epi, bool red, EoArgs e, unsigned zlen, unsigned row
   return 0;
 }
 
+// one launch of the hop with z chunks of zlen planes (1: one site per lane);
+// two: two rows per lane, rows: rows or row pairs
 template <int PS, int PG>
-static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
-                        int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st,
-                        int *grid_out) {
-  const SpinorIn<PS> si = SpinorIn<PS>::of(in, ri);
-  const SpinorIn<PS> sb = b ? SpinorIn<PS>::of(b, rb) : si;
-  const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
-  const GaugeIn<PG> gu = GaugeIn<PG>::of(g);
-#define QH_E(E, RD) k_eo_hop<PS, PG, E, RD><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
-  if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16) {
-    if ((int)e.d.count >= eo_mma_min_rhs() && !red && (epi == EO_MINV || epi == EO_SCHUR))
-      return eo_mma_launch(epi, e, out, ro, in, ri, g, b, rb, st);
-  }
-  {
-    const bool two = PS != QCD_HIP_F64 && e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) &&
-                     eo_two_rhs();
-    const unsigned rows = two ? (e.d.count + 1) / 2 : e.d.count;
-    const unsigned zlen = eo_zlen(e, rows);
-    if (zlen > 1) {
-      if (red && grid_out) *grid_out = (int)(((e.d.L[0] * e.d.L[1] * e.d.L[2] / 2 + kBlock - 1) / kBlock) *
-                                             ((e.d.L[3] + zlen - 1) / zlen));
-      if constexpr (PS != QCD_HIP_F64) {
-        if (two) return eo_zm_launch<PS, PG, 2>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
-      }
-      return eo_zm_launch<PS, PG, 1>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
+static int eo_launch_z(int epi, bool red, const EoArgs &e, unsigned zlen, bool two, unsigned rows, dim3 gr,
+                       const SpinorIn<PS> &si, const GaugeIn<PG> &gu, const SpinorOut<PS> &so, const SpinorIn<PS> &sb,
+                       double *partials, hipStream_t st) {
+  if (zlen > 1) {
+    if constexpr (PS != QCD_HIP_F64) {
+      if (two) return eo_zm_launch<PS, PG, 2>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
     }
+    return eo_zm_launch<PS, PG, 1>(epi, red, e, zlen, rows, si, gu, so, sb, partials, st);
   }
   if constexpr (PS != QCD_HIP_F64) {
-    if (e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) && eo_two_rhs()) {
-      const dim3 g2(e.d.nblk * ((e.d.count + 1) / 2));
+    if (two) {
+      const dim3 g2(e.d.nblk * rows);
       if (epi == EO_MINV) k_eo_hop2<PS, PG, EO_MINV><<<g2, kBlock, 0, st>>>(e, si, gu, so, sb);
       else k_eo_hop2<PS, PG, EO_SCHUR><<<g2, kBlock, 0, st>>>(e, si, gu, so, sb);
       QH_CHECK(hipGetLastError());
       return 0;
     }
   }
+#define QH_E(E, RD) k_eo_hop<PS, PG, E, RD><<<gr, kBlock, 0, st>>>(e, si, gu, so, sb, partials)
   switch (epi * 2 + (red ? 1 : 0)) {
     case 0: QH_E(EO_MINV, false); break;
     case 2: QH_E(EO_SCHUR, false); break;
@@ -627,6 +621,95 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
 #undef QH_E
   QH_CHECK(hipGetLastError());
   return 0;
+}
+
+// z chunks tuned per (lattice extents, spinor precision, rows per lane, rows
+// or row pairs per launch): the first launch of a shape that may run twice
+// (not under graph capture, out aliasing neither in nor b) times every chunk
+// length and keeps the fastest.  All give bit-identical results, and the
+// timing is short: the runs where the shape of the grid against the 256 CUs
+// decides (24^3x48, one FP64 rhs: 289 us at 6 steps, 423 at 5, 383 at 1) make
+// a formula unreliable.
+struct EoZKey {
+  int L[4], ps, nr;
+  unsigned rows;
+  bool operator<(const EoZKey &o) const {
+    return std::tie(L[0], L[1], L[2], L[3], ps, nr, rows) < std::tie(o.L[0], o.L[1], o.L[2], o.L[3], o.ps, o.nr, o.rows);
+  }
+};
+static std::map<EoZKey, unsigned> &eo_ztuned() {
+  static std::map<EoZKey, unsigned> m;
+  return m;
+}
+
+template <int PS, int PG>
+static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_field out, int ro, qcd_hip_field in,
+                        int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t st,
+                        int *grid_out) {
+  const SpinorIn<PS> si = SpinorIn<PS>::of(in, ri);
+  const SpinorIn<PS> sb = b ? SpinorIn<PS>::of(b, rb) : si;
+  const SpinorOut<PS> so = SpinorOut<PS>::of(out, ro);
+  const GaugeIn<PG> gu = GaugeIn<PG>::of(g);
+  if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16) {
+    if ((int)e.d.count >= eo_mma_min_rhs() && !red && (epi == EO_MINV || epi == EO_SCHUR))
+      return eo_mma_launch(epi, e, out, ro, in, ri, g, b, rb, st);
+  }
+  const bool two = PS != QCD_HIP_F64 && e.d.count >= 2 && !red && (epi == EO_MINV || epi == EO_SCHUR) &&
+                   eo_two_rhs();
+  const unsigned rows = two ? (e.d.count + 1) / 2 : e.d.count;
+  unsigned zlen = 1;
+  if (eo_zlen_setting().load() > 0) {
+    zlen = eo_zlen(e, rows);
+  } else {
+    std::lock_guard<std::recursive_mutex> lock(state_mutex());
+    const EoZKey key{{(int)e.d.L[0], (int)e.d.L[1], (int)e.d.L[2], (int)e.d.L[3]}, PS, two ? 2 : 1, rows};
+    auto it = eo_ztuned().find(key);
+    if (it != eo_ztuned().end()) {
+      zlen = it->second;
+    } else {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      QH_CHECK(hipStreamIsCapturing(st, &cs));
+      if (cs != hipStreamCaptureStatusNone || out == in || out == b) {
+        zlen = eo_zlen(e, rows);   // no timing possible: the work-size rule
+      } else {
+        hipEvent_t e0, e1;
+        QH_CHECK(hipEventCreate(&e0));
+        QH_CHECK(hipEventCreate(&e1));
+        const int reps = (double)rows * e.Vh > 4.0e6 ? 2 : 5;
+        float best = 1e30f;
+        int rc = 0;
+        for (unsigned z : {1u, 2u, 3u, 4u, 6u, 8u, 12u, 16u, 24u}) {
+          if (z > 1 && eo_zlen_fits(e, z) != z) continue;
+          rc = eo_launch_z<PS, PG>(epi, red, e, z, two, rows, gr, si, gu, so, sb, partials, st);
+          if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = 2;
+          for (int r = 0; r < reps && !rc; r++)
+            rc = eo_launch_z<PS, PG>(epi, red, e, z, two, rows, gr, si, gu, so, sb, partials, st);
+          float ms = 0.f;
+          if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = 2;
+          if (rc) break;
+          if (getenv("QCD_HIP_TUNE_VERBOSE"))
+            fprintf(stderr, "eo tune L=%dx%dx%dx%d ps=%d rows/lane=%d rows=%u zlen=%u: %.2f us\n", key.L[0], key.L[1],
+                    key.L[2], key.L[3], PS, key.nr, rows, z, ms * 1e3f / reps);
+          if (ms < best) {
+            best = ms;
+            zlen = z;
+          }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) {
+          if (rc == 2) set_error("eo: timing the z chunks failed: %s", hipGetErrorString(hipGetLastError()));
+          return rc;
+        }
+        eo_ztuned()[key] = zlen;
+      }
+    }
+  }
+  if (zlen > 1 && red && grid_out)
+    *grid_out = (int)(((e.d.L[0] * e.d.L[1] * e.d.L[2] / 2 + kBlock - 1) / kBlock) * ((e.d.L[3] + zlen - 1) / zlen));
+  return eo_launch_z<PS, PG>(epi, red, e, zlen, two, rows, gr, si, gu, so, sb, partials, st);
 }
 
 // Batched EO hop: output sites of parity par; out/in/b are half-lattice

@@ -473,6 +473,17 @@ int run_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream
   if (!S->prm.graph) return gcr_cycle<P>(S, count, kappa, mu, st);
   if (!S->cycle || S->cycle_count != count || S->cycle_kappa != kappa || S->cycle_mu != mu) {
     S->drop_cycle();
+    if (S->prm.eo) {
+      // the cycle's first matvec once, eagerly: hop launch shapes seen for the
+      // first time are tuned here, which cannot happen under capture (both
+      // results are recomputed by the cycle)
+      const double m = 1.0 / (2.0 * kappa) - 4.0;
+      const int n = S->prm.n_restart;
+      QH_TRY(eo_launch(EO_MINV, false, S->te, 0, 1, S->Rs, 0, n, S->gin, nullptr, 0, 0, S->ids, count, m, mu, 0,
+                       nullptr, st, nullptr));
+      QH_TRY(eo_launch(EO_SCHUR, false, S->Ws, 0, n, S->te, 0, 1, S->gin, S->Rs, 0, n, S->ids, count, m, mu, 1,
+                       nullptr, st, nullptr));
+    }
     QH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     const int rc = gcr_cycle<P>(S, count, kappa, mu, st);
     hipGraph_t graph = nullptr;
