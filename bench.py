@@ -240,6 +240,10 @@ def solves_leg(Q, prec, batch, n_restart, L=(32, 16, 16, 16), kappa=0.130, eps=0
     x1 = Q.Spinor(lat)
     S1 = Q.Solver(lat, g, n_restart=n_restart, tol=1e-4, inner_prec=prec, batch=1, eo=eo)
     S1.solve(x1, b1, kappa)   # tunes the operators outside the timed region
+    if batch > 1:   # the even-odd hops are tuned per batch size: one short solve of the timed shape
+        Sw = Q.Solver(lat, g, n_restart=n_restart, tol=1e-2, inner_prec=prec, batch=batch, eo=eo, max_outer=1)
+        Sw.solve(x, b, kappa, check=False)
+        Sw.close()
     Q.synchronize()
     t = time.perf_counter()
     st = S.solve(x, b, kappa)
@@ -301,6 +305,10 @@ def pipeline_48c96_leg(Q, batches=None, seed=5005, keep=None):
     x1 = Q.Spinor(lat)
     Q.Solver(lat, g, n_restart=32, tol=1e-2, inner_prec="f16", batch=1, eo=1).solve(x1, b1, kappa)
     del b1, x1
+    for bt in sorted(set(batches.values())):   # the even-odd hops are tuned per batch size
+        Sw = Q.Solver(lat, g, n_restart=12, tol=1e-2, inner_prec="f16", batch=bt, eo=1, max_outer=1)
+        Sw.solve(x, b, kappa, check=False)
+        Sw.close()
     solves = {}
     for n in (32, 12):   # invert.ini:8's N_restart, and the shorter cycle
         S = Q.Solver(lat, g, n_restart=n, tol=1e-10, inner_prec="f16", batch=batches[n], max_outer=20000, eo=1)

@@ -624,12 +624,12 @@ static int eo_launch_z(int epi, bool red, const EoArgs &e, unsigned zlen, bool t
 }
 
 // z chunks tuned per (lattice extents, spinor precision, rows per lane, rows
-// or row pairs per launch): the first launch of a shape that may run twice
-// (not under graph capture, out aliasing neither in nor b) times every chunk
-// length and keeps the fastest.  All give bit-identical results, and the
-// timing is short: the runs where the shape of the grid against the 256 CUs
-// decides (24^3x48, one FP64 rhs: 289 us at 6 steps, 423 at 5, 383 at 1) make
-// a formula unreliable.
+// or row pairs per launch): the first launch of a shape with at least 2^17
+// (row, half-site) items that may run twice (not under graph capture, out
+// aliasing neither in nor b) times chunks of 1 to 12 planes and keeps the
+// fastest; smaller launches keep one site per lane.  All give bit-identical
+// results.  Timing rather than a formula: how the grid fills the 256 CUs
+// decides (24^3x48, one FP64 rhs: 289 us at 6 steps, 423 at 5, 383 at 1).
 struct EoZKey {
   int L[4], ps, nr;
   unsigned rows;
@@ -658,8 +658,8 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
                    eo_two_rhs();
   const unsigned rows = two ? (e.d.count + 1) / 2 : e.d.count;
   unsigned zlen = 1;
-  if (eo_zlen_setting().load() > 0) {
-    zlen = eo_zlen(e, rows);
+  if (eo_zlen_setting().load() > 0 || (double)rows * e.Vh < 131072.0) {
+    zlen = eo_zlen(e, rows);   // as set, or too small a launch to gain from the z-march
   } else {
     std::lock_guard<std::recursive_mutex> lock(state_mutex());
     const EoZKey key{{(int)e.d.L[0], (int)e.d.L[1], (int)e.d.L[2], (int)e.d.L[3]}, PS, two ? 2 : 1, rows};
@@ -678,7 +678,7 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
         const int reps = (double)rows * e.Vh > 4.0e6 ? 2 : 5;
         float best = 1e30f;
         int rc = 0;
-        for (unsigned z : {1u, 2u, 3u, 4u, 6u, 8u, 12u, 16u, 24u}) {
+        for (unsigned z : {1u, 2u, 3u, 4u, 6u, 8u, 12u}) {
           if (z > 1 && eo_zlen_fits(e, z) != z) continue;
           rc = eo_launch_z<PS, PG>(epi, red, e, z, two, rows, gr, si, gu, so, sb, partials, st);
           if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = 2;
