@@ -637,8 +637,11 @@ struct EoZKey {
     return std::tie(L[0], L[1], L[2], L[3], ps, nr, rows) < std::tie(o.L[0], o.L[1], o.L[2], o.L[3], o.ps, o.nr, o.rows);
   }
 };
-static std::map<EoZKey, unsigned> &eo_ztuned() {
-  static std::map<EoZKey, unsigned> m;
+struct EoZVal {
+  unsigned zlen, rgroup;
+};
+static std::map<EoZKey, EoZVal> &eo_ztuned() {
+  static std::map<EoZKey, EoZVal> m;
   return m;
 }
 
@@ -658,6 +661,7 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
                    eo_two_rhs();
   const unsigned rows = two ? (e.d.count + 1) / 2 : e.d.count;
   unsigned zlen = 1;
+  EoArgs et = e;   // et.rgroup: tuned with the z chunk
   if (eo_zlen_setting().load() > 0 || (double)rows * e.Vh < 131072.0) {
     zlen = eo_zlen(e, rows);   // as set, or too small a launch to gain from the z-march
   } else {
@@ -665,7 +669,8 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
     const EoZKey key{{(int)e.d.L[0], (int)e.d.L[1], (int)e.d.L[2], (int)e.d.L[3]}, PS, two ? 2 : 1, rows};
     auto it = eo_ztuned().find(key);
     if (it != eo_ztuned().end()) {
-      zlen = it->second;
+      zlen = it->second.zlen;
+      et.rgroup = it->second.rgroup;
     } else {
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
       QH_CHECK(hipStreamIsCapturing(st, &cs));
@@ -678,24 +683,37 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
         const int reps = (double)rows * e.Vh > 4.0e6 ? 2 : 5;
         float best = 1e30f;
         int rc = 0;
+        // batch rows per group (eo_row_of): 4 suits one site per lane; with
+        // the z-march larger groups can share the links better (24^3x48, 12
+        // rhs: FP64 165 -> 146, FP32 82 -> 75 us per rhs with 12)
+        const bool rg_set = getenv("QCD_HIP_EO_RGROUP") != nullptr;   // a set group size is kept
+        const unsigned rg4 = rg_set ? e.rgroup : std::min(4u, e.d.count);
+        const unsigned rg12 = rg_set ? e.rgroup : std::min(12u, e.d.count);
         for (unsigned z : {1u, 2u, 3u, 4u, 6u, 8u, 12u}) {
           if (z > 1 && eo_zlen_fits(e, z) != z) continue;
-          rc = eo_launch_z<PS, PG>(epi, red, e, z, two, rows, gr, si, gu, so, sb, partials, st);
-          if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = 2;
-          for (int r = 0; r < reps && !rc; r++)
-            rc = eo_launch_z<PS, PG>(epi, red, e, z, two, rows, gr, si, gu, so, sb, partials, st);
-          float ms = 0.f;
-          if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = 2;
-          if (rc) break;
-          if (getenv("QCD_HIP_TUNE_VERBOSE"))
-            fprintf(stderr, "eo tune L=%dx%dx%dx%d ps=%d rows/lane=%d rows=%u zlen=%u: %.2f us\n", key.L[0], key.L[1],
-                    key.L[2], key.L[3], PS, key.nr, rows, z, ms * 1e3f / reps);
-          if (ms < best) {
-            best = ms;
-            zlen = z;
+          for (unsigned rg : {rg4, rg12}) {
+            EoArgs ec = e;
+            ec.rgroup = rg;
+            rc = eo_launch_z<PS, PG>(epi, red, ec, z, two, rows, gr, si, gu, so, sb, partials, st);
+            if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = 2;
+            for (int r = 0; r < reps && !rc; r++)
+              rc = eo_launch_z<PS, PG>(epi, red, ec, z, two, rows, gr, si, gu, so, sb, partials, st);
+            float ms = 0.f;
+            if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+              rc = 2;
+            if (rc) break;
+            if (getenv("QCD_HIP_TUNE_VERBOSE"))
+              fprintf(stderr, "eo tune L=%dx%dx%dx%d ps=%d rows/lane=%d rows=%u zlen=%u rgroup=%u: %.2f us\n",
+                      key.L[0], key.L[1], key.L[2], key.L[3], PS, key.nr, rows, z, rg, ms * 1e3f / reps);
+            if (ms < best) {
+              best = ms;
+              zlen = z;
+              et.rgroup = rg;
+            }
+            if (rg4 == rg12) break;   // one group size
           }
+          if (rc) break;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -703,13 +721,13 @@ static int eo_launch_pp(int epi, bool red, const EoArgs &e, dim3 gr, qcd_hip_fie
           if (rc == 2) set_error("eo: timing the z chunks failed: %s", hipGetErrorString(hipGetLastError()));
           return rc;
         }
-        eo_ztuned()[key] = zlen;
+        eo_ztuned()[key] = EoZVal{zlen, et.rgroup};
       }
     }
   }
   if (zlen > 1 && red && grid_out)
     *grid_out = (int)(((e.d.L[0] * e.d.L[1] * e.d.L[2] / 2 + kBlock - 1) / kBlock) * ((e.d.L[3] + zlen - 1) / zlen));
-  return eo_launch_z<PS, PG>(epi, red, e, zlen, two, rows, gr, si, gu, so, sb, partials, st);
+  return eo_launch_z<PS, PG>(epi, red, et, zlen, two, rows, gr, si, gu, so, sb, partials, st);
 }
 
 // Batched EO hop: output sites of parity par; out/in/b are half-lattice
