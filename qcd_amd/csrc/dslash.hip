@@ -47,7 +47,7 @@ __device__ __forceinline__ void local_terms(Cx<R> acc[4][3], const typename Prec
 template <int PS, int PG, int EPI, bool RED, int OCC>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
-         double *__restrict__ partials) {
+         double *__restrict__ partials, typename SplitOf<EPI>::type split) {
   using R = typename Prec<PS>::R;
   const Site st = site_of_lane(a);
   const unsigned s = st.s;
@@ -91,7 +91,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
 #pragma unroll
       for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     }
-    out.template store<true>(s, o);
+    store_site<EPI, PS>(out, split_row(split, a.ids ? (unsigned)a.ids[blockIdx.y] : blockIdx.y), st, o);
   }
   if (RED) block_reduce_store(red, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -108,7 +108,7 @@ k_dslash(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all
 template <int PS, int PG, int EPI, bool RED, int OCC, bool LDSC = false>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
-            double *__restrict__ partials) {
+            double *__restrict__ partials, typename SplitOf<EPI>::type split) {
   using R = typename Prec<PS>::R;
   using T = typename Prec<PS>::T;
   __shared__ T zfwd[LDSC ? 12 : 1][kBlock];
@@ -117,6 +117,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, br.row);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
   const SpinorOut<PS> &out = rv.out;
+  const auto srow = split_row(split, a.ids ? (unsigned)a.ids[br.row] : br.row);
   const unsigned lb = br.blk;
   const unsigned cb = lb % a.col_blocks, zc = lb / a.col_blocks;
   const unsigned P3 = a.L[0] * a.L[1] * a.L[2];
@@ -190,7 +191,7 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 #pragma unroll
         for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
       }
-      out.template store<true>(s, o);
+      store_site<EPI, PS>(out, srow, st, o);
     }
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
@@ -340,7 +341,7 @@ inline size_t ztile_lds_bytes(int ps, unsigned T, unsigned xb) {
 template <int PS, int PG, int EPI, bool RED, int OCC>
 __global__ void __launch_bounds__(kBlock) QH_WAVES(OCC)
 k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_all, SpinorIn<PS> b_all,
-            double *__restrict__ partials) {
+            double *__restrict__ partials, typename SplitOf<EPI>::type split) {
   using R = typename Prec<PS>::R;
   using T = typename Prec<PS>::T;
   extern __shared__ __align__(16) unsigned char qh_dyn_lds[];
@@ -352,6 +353,7 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
   const RhsView<PS> rv = rhs_view(a, in_all, out_all, b_all, br.row);
   const SpinorIn<PS> &in = rv.in, &b = rv.b;
   const SpinorOut<PS> &out = rv.out;
+  const auto srow = split_row(split, a.ids ? (unsigned)a.ids[br.row] : br.row);
   const unsigned lb = br.blk;
   const unsigned g = lb % a.nxg, r1 = lb / a.nxg;
   const unsigned y = r1 % Ly, zc = r1 / Ly;
@@ -464,7 +466,7 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 #pragma unroll
         for (int k = 0; k < 12; k++) red += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
       }
-      out.template store<true>(s, o);
+      store_site<EPI, PS>(out, srow, st, o);
     }
   }
   if (RED) block_reduce_store(red, partials, br.row * a.nblk + br.blk);
@@ -472,8 +474,18 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
 
 template <int PS, int PG, int EPI, bool RED, int OCC>
 static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
-                      qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
+                      qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream,
+                      const void *split_host) {
   SpinorIn<PS> bi = b ? SpinorIn<PS>::of(b, rb) : SpinorIn<PS>::of(in, ri);
+  typename SplitOf<EPI>::type sp{};
+  if constexpr (epi_eo_split<EPI>()) {
+    if (!split_host || (a.map != MAP_ZMARCH && a.map != MAP_ZMARCH_LDS && a.map != MAP_ZTILE && a.map != MAP_XCD &&
+                        a.map != MAP_SLAB)) {
+      set_error("dslash: the even-odd split residual needs its half fields and a one-rhs-per-lane traversal");
+      return 1;
+    }
+    sp = *(const typename SplitOf<EPI>::type *)split_host;
+  }
   // z-march traversals interleave the batch rows into grid.x (see block_row)
   if (a.map == MAP_MMA) {
     if constexpr (PS == QCD_HIP_F16 && PG == QCD_HIP_F16 && !RED && (EPI == EPI_D || EPI == EPI_Q)) {
@@ -487,10 +499,10 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
   const dim3 gr(zmarch ? (unsigned)grid * a.count : (unsigned)grid, zmarch ? 1u : (unsigned)a.count);
   if (a.map == MAP_ZMARCH)
     k_dslash_zm<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                                  SpinorOut<PS>::of(out, ro), bi, partials);
+                                                                  SpinorOut<PS>::of(out, ro), bi, partials, sp);
   else if (a.map == MAP_ZMARCH_LDS)
     k_dslash_zm<PS, PG, EPI, RED, OCC, true><<<gr, kBlock, 0, stream>>>(
-        a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g), SpinorOut<PS>::of(out, ro), bi, partials);
+        a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g), SpinorOut<PS>::of(out, ro), bi, partials, sp);
   else if (a.map == MAP_ZMARCH2) {
     if constexpr (PS != QCD_HIP_F64 && !RED && (EPI == EPI_D || EPI == EPI_Q))
       k_dslash_zm2<PS, PG, EPI, OCC><<<dim3((unsigned)grid * a.pairs), kBlock, 0, stream>>>(
@@ -509,19 +521,20 @@ static int launch_occ(const DslashArgs &a, int grid, qcd_hip_field out, int ro, 
       lds_set.store(lds);
     }
     k_dslash_zt<PS, PG, EPI, RED, OCC><<<gr, a.lanes, lds, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                                     SpinorOut<PS>::of(out, ro), bi, partials);
+                                                                     SpinorOut<PS>::of(out, ro), bi, partials, sp);
   }
   else
     k_dslash<PS, PG, EPI, RED, OCC><<<gr, kBlock, 0, stream>>>(a, SpinorIn<PS>::of(in, ri), GaugeIn<PG>::of(g),
-                                                               SpinorOut<PS>::of(out, ro), bi, partials);
+                                                               SpinorOut<PS>::of(out, ro), bi, partials, sp);
   QH_CHECK(hipGetLastError());
   return 0;
 }
 
 template <int PS, int PG, int EPI, bool RED>
 static int launch_one(const DslashArgs &a, int occ, int grid, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
-                      qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream) {
-#define QH_O(N) launch_occ<PS, PG, EPI, RED, N>(a, grid, out, ro, in, ri, g, b, rb, partials, stream)
+                      qcd_hip_field g, qcd_hip_field b, int rb, double *partials, hipStream_t stream,
+                      const void *split_host) {
+#define QH_O(N) launch_occ<PS, PG, EPI, RED, N>(a, grid, out, ro, in, ri, g, b, rb, partials, stream, split_host)
   switch (occ) {
     case 2: return QH_O(2);
     case 3: return QH_O(3);
@@ -534,8 +547,8 @@ static int launch_one(const DslashArgs &a, int occ, int grid, qcd_hip_field out,
 template <int PS, int PG>
 static int launch_epi(int epi, bool red, const DslashArgs &a, int occ, int grid, qcd_hip_field out, int ro,
                       qcd_hip_field in, int ri, qcd_hip_field g, qcd_hip_field b, int rb, double *partials,
-                      hipStream_t stream) {
-#define QH_L(E, RD) launch_one<PS, PG, E, RD>(a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream)
+                      hipStream_t stream, const void *split_host = nullptr) {
+#define QH_L(E, RD) launch_one<PS, PG, E, RD>(a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream, split_host)
   switch (epi * 2 + (red ? 1 : 0)) {
     case 0: return QH_L(EPI_D, false);
     case 1: return QH_L(EPI_D, true);
@@ -544,9 +557,15 @@ static int launch_epi(int epi, bool red, const DslashArgs &a, int occ, int grid,
     case 4: return QH_L(EPI_RES_D, false);
     case 5: return QH_L(EPI_RES_D, true);
     case 6: return QH_L(EPI_RES_Q, false);
-    default: return QH_L(EPI_RES_Q, true);
+    case 7: return QH_L(EPI_RES_Q, true);
+    case 9:   // the even-odd split residual: FP64 fields, with the |r|^2 reduction
+      if constexpr (PS == QCD_HIP_F64 && PG == QCD_HIP_F64) return QH_L(EPI_RES_D_EO16, true);
+      break;
+    default: break;
   }
 #undef QH_L
+  set_error("dslash: unsupported epilogue %d (reduction %d) for spinor precision %d", epi, (int)red, PS);
+  return 1;
 }
 
 DslashArgs make_args(qcd_hip_lattice lat, double m, double tm) {
@@ -671,7 +690,7 @@ static const Batch kOne{nullptr, 1, 0, 0, 0};
 
 static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, int ro, qcd_hip_field in, int ri,
                       qcd_hip_field g, qcd_hip_field b, int rb, double m, double tm, double *partials,
-                      hipStream_t stream, int *grid_out, const Batch &bt = kOne) {
+                      hipStream_t stream, int *grid_out, const Batch &bt = kOne, const void *split = nullptr) {
   DslashArgs a = make_args(in->lat, m, tm);
   const int grid = set_traversal(a, map, in->prec);
   a.ids = bt.ids;
@@ -703,23 +722,23 @@ static int launch_cfg(int epi, bool red, int map, int occ, qcd_hip_field out, in
     if (grid_out) *grid_out = gz;
     const int ps = in->prec, pg = g->prec;
     if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
-      return launch_epi<QCD_HIP_F64, QCD_HIP_F64>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+      return launch_epi<QCD_HIP_F64, QCD_HIP_F64>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream, split);
     if (ps == QCD_HIP_F32 && pg == QCD_HIP_F32)
-      return launch_epi<QCD_HIP_F32, QCD_HIP_F32>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+      return launch_epi<QCD_HIP_F32, QCD_HIP_F32>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream, split);
     if (ps == QCD_HIP_F32 && pg == QCD_HIP_F16)
-      return launch_epi<QCD_HIP_F32, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
-    return launch_epi<QCD_HIP_F16, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream);
+      return launch_epi<QCD_HIP_F32, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream, split);
+    return launch_epi<QCD_HIP_F16, QCD_HIP_F16>(epi, red, a, occ, gz, out, ro, in, ri, g, b, rb, partials, stream, split);
   }
   if (grid_out) *grid_out = grid;
   const int ps = in->prec, pg = g->prec;
   if (ps == QCD_HIP_F64 && pg == QCD_HIP_F64)
-    return launch_epi<QCD_HIP_F64, QCD_HIP_F64>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream);
+    return launch_epi<QCD_HIP_F64, QCD_HIP_F64>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream, split);
   if (ps == QCD_HIP_F32 && pg == QCD_HIP_F32)
-    return launch_epi<QCD_HIP_F32, QCD_HIP_F32>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream);
+    return launch_epi<QCD_HIP_F32, QCD_HIP_F32>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream, split);
   if (ps == QCD_HIP_F32 && pg == QCD_HIP_F16)
-    return launch_epi<QCD_HIP_F32, QCD_HIP_F16>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream);
+    return launch_epi<QCD_HIP_F32, QCD_HIP_F16>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream, split);
   if (ps == QCD_HIP_F16 && pg == QCD_HIP_F16)
-    return launch_epi<QCD_HIP_F16, QCD_HIP_F16>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream);
+    return launch_epi<QCD_HIP_F16, QCD_HIP_F16>(epi, red, a, occ, grid, out, ro, in, ri, g, b, rb, partials, stream, split);
   set_error("dslash: unsupported precision pair (spinor %d, gauge %d)", ps, pg);
   return 1;
 }
@@ -886,6 +905,32 @@ int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs
   const int map = two ? t.bmap : have ? t.map : kDefaultMap, occ = two ? t.bocc : have ? t.occ : 0;
   const Batch bt{ids, count, in_rs, out_rs, b_rs};
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out, bt);
+}
+
+// r = b - D x of a batch (FP64 fields, as dslash_launch_batch with EPI_RES_D
+// and the |r|^2 partials), stored split by parity into the FP16 half fields
+// ev, od and tev = M^-1 ev (vector ids[y] of each for row y) instead of an
+// FP64 field: the FP64 residual never goes to memory.  (An FP32 form of the
+// same epilogue spilled registers in these kernels: FP32 inner solves keep the
+// FP64 residual and a split kernel.)
+int dslash_residual_eo_split(qcd_hip_field x, int x0, int x_rs, qcd_hip_field g, qcd_hip_field b, int rb, int b_rs,
+                             const int *ids, int count, double m, double tm, qcd_hip_field ev, qcd_hip_field od,
+                             qcd_hip_field tev, double *partials, hipStream_t stream, int *grid_out) {
+  if (count < 1) return 0;
+  if (x->prec != QCD_HIP_F64 || g->prec != QCD_HIP_F64 || b->prec != QCD_HIP_F64 || ev->prec != QCD_HIP_F16 ||
+      od->prec != QCD_HIP_F16 || tev->prec != QCD_HIP_F16) {
+    set_error("dslash: the split residual takes FP64 x, b and gauge fields and FP16 half fields");
+    return 1;
+  }
+  TuneVal t;
+  const bool have = tune_get(key_of(x, g), &t);
+  const bool two = have && count >= 2 && t.bmap != 0;
+  const int map = two ? t.bmap : have ? t.map : kDefaultMap, occ = two ? t.bocc : have ? t.occ : 0;
+  const Batch bt{ids, count, x_rs, 0, b_rs};
+  const double a4 = m + 4.0, den = a4 * a4 + tm * tm;   // M^-1 as make_eo_args forms it
+  const EoSplit<QCD_HIP_F16> sp{SpinorOut<QCD_HIP_F16>::of(ev, 0), SpinorOut<QCD_HIP_F16>::of(od, 0),
+                                SpinorOut<QCD_HIP_F16>::of(tev, 0), a4 / den, -tm / den};
+  return launch_cfg(EPI_RES_D_EO16, true, map, occ, x, 0, x, x0, g, b, rb, m, tm, partials, stream, grid_out, bt, &sp);
 }
 
 // Tune (once per key) before the first use, outside any graph capture.

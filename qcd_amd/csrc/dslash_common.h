@@ -723,7 +723,14 @@ __device__ __forceinline__ void hop_dir(Cx<R> acc[4][3], const DslashArgs &a, co
 }
 
 // ------------------------------------------------------------ epilogues
-enum { EPI_D = 0, EPI_Q = 1, EPI_RES_D = 2, EPI_RES_Q = 3 };
+// EPI_RES_D_EO16: the residual b - D x, stored not as one FP64 field but split
+// by parity into the FP16 half fields of an even-odd solve's inner
+// precision, with M^-1 of the even half as well (eo_split_store)
+enum { EPI_D = 0, EPI_Q = 1, EPI_RES_D = 2, EPI_RES_Q = 3, EPI_RES_D_EO16 = 4 };
+template <int EPI>
+constexpr bool epi_eo_split() {
+  return EPI == EPI_RES_D_EO16;
+}
 
 template <int EPI, int PS, typename R>
 __device__ __forceinline__ void apply_epilogue(Cx<R> o[12], const SpinorIn<PS> &b, unsigned s) {
@@ -733,11 +740,78 @@ __device__ __forceinline__ void apply_epilogue(Cx<R> o[12], const SpinorIn<PS> &
 #pragma unroll
     for (int k = 6; k < 12; k++) o[k] = ipow<3>(o[k]);
   }
-  if constexpr (EPI == EPI_RES_D || EPI == EPI_RES_Q) {
+  if constexpr (EPI == EPI_RES_D || EPI == EPI_RES_Q || epi_eo_split<EPI>()) {
     Cx<R> bv[12];
     b.load(s, bv);
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = Cx<R>{bv[k].re - o[k].re, bv[k].im - o[k].im};
+  }
+}
+
+// The split targets of EPI_RES_D_EO16 (kernel argument; NoSplit for every
+// other epilogue): row slot b's residual at site s goes to entry s >> 1
+// of vector b of ev (even s) or od (odd s) in precision P, and for even s
+// M^-1 of it, M = m + 4 + i tm g5, to tev -- the even-odd split of an outer
+// step, from the FP64 values with the same arithmetic as a split kernel that
+// reads them back.
+struct NoSplit {};
+template <int P>
+struct EoSplit {
+  SpinorOut<P> ev, od, tev;
+  double mi_re, mi_im;   // 1 / (m + 4 + i tm)
+};
+template <int EPI>
+struct SplitOf {
+  using type = NoSplit;
+};
+template <>
+struct SplitOf<EPI_RES_D_EO16> {
+  using type = EoSplit<QCD_HIP_F16>;
+};
+
+// the split targets of batch vector slot (the row's views, formed once)
+template <int P>
+__device__ __forceinline__ EoSplit<P> split_row(const EoSplit<P> &sp, unsigned slot) {
+  return EoSplit<P>{sp.ev.advanced(slot), sp.od.advanced(slot), sp.tev.advanced(slot), sp.mi_re, sp.mi_im};
+}
+__device__ __forceinline__ NoSplit split_row(const NoSplit &sp, unsigned) { return sp; }
+
+// sp: the row's views (split_row)
+template <int P>
+__device__ __forceinline__ void eo_split_store(const EoSplit<P> &sp, unsigned s, unsigned par, const Cx<double> o[12]) {
+  using R = typename Prec<P>::R;
+  Cx<R> v[12];
+  if (par) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = Cx<R>{(R)o[k].re, (R)o[k].im};
+    sp.od.template store<true>(s >> 1, v);
+  } else {
+    const double are = sp.mi_re, aim = sp.mi_im;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      const double im = k < 6 ? aim : -aim;   // g5 = diag(1, 1, -1, -1)
+      v[k] = Cx<R>{(R)fma(are, o[k].re, -(im * o[k].im)), (R)fma(are, o[k].im, im * o[k].re)};
+    }
+    sp.tev.template store<true>(s >> 1, v);
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = Cx<R>{(R)o[k].re, (R)o[k].im};
+    sp.ev.template store<true>(s >> 1, v);
+  }
+}
+
+// the residual of a site stored as the epilogue asks: out, or the even-odd
+// split (PS = FP64 only; split: the row's views)
+template <int EPI, int PS, typename R>
+__device__ __forceinline__ void store_site(const SpinorOut<PS> &out, const typename SplitOf<EPI>::type &split,
+                                           const Site &st, const Cx<R> o[12]) {
+  if constexpr (epi_eo_split<EPI>()) {
+    static_assert(PS == QCD_HIP_F64, "the even-odd split reads the FP64 residual");
+    // keep the next z step's loads below the split's conversions and stores
+    __builtin_amdgcn_sched_barrier(0);
+    eo_split_store(split, st.s, (st.t + st.x + st.y + st.z) & 1u, o);
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    out.template store<true>(st.s, o);
   }
 }
 
@@ -772,6 +846,12 @@ int dslash_mma_launch(int epi, DslashArgs a, qcd_hip_field out, int ro, qcd_hip_
 int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_field in, int ri, int in_rs,
                         qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m,
                         double tm, double *partials, hipStream_t stream, int *grid_out);
+// r = b - D x of a batch of FP64 fields with |r|^2 partials, stored split by
+// parity into FP16 half fields ev, od and tev = M^-1 ev (vector ids[y] of each
+// for batch row y), M = m + 4 + i tm g5
+int dslash_residual_eo_split(qcd_hip_field x, int x0, int x_rs, qcd_hip_field g, qcd_hip_field b, int rb, int b_rs,
+                             const int *ids, int count, double m, double tm, qcd_hip_field ev, qcd_hip_field od,
+                             qcd_hip_field tev, double *partials, hipStream_t stream, int *grid_out);
 DslashArgs make_args(qcd_hip_lattice lat, double m, double tm);
 int dslash_check(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g);
 // asynchronous single-rhs launch; *grid_out (if given) receives the number of

@@ -393,9 +393,10 @@ __global__ void __launch_bounds__(kBlock) k_eo_scale64(EoArgs e, SpinorSet<P> in
   out.store(b, h, o);
 }
 
-// Inner-precision outer step of a mixed even-odd solve, one lane per site:
-// the FP64 residual f0 + slot of the full lattice -> its even half ev, its odd
-// half od and tev = M^-1 ev, all three in precision P
+// Inner-precision outer step of a mixed even-odd solve with FP32 inner
+// vectors, one lane per site: the FP64 residual f0 + slot of the full lattice
+// -> its even half ev, its odd half od and tev = M^-1 ev, all three in
+// precision P (FP16 solves get the halves from the residual kernel itself)
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_eo_split_minv(EoArgs e, SpinorSet<QCD_HIP_F64> full, int f0,
                                                           SpinorSet<P> ev, SpinorSet<P> od, SpinorSet<P> tev,

@@ -94,8 +94,10 @@ int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_
               qcd_hip_field g, qcd_hip_field b, int rb, int b_rs, const int *ids, int count, double m, double tm,
               unsigned par, double *partials, hipStream_t st, int *grid_out);
 // The outer step of a mixed even-odd solve in the inner precision (eo.hip):
-// FP64 residual -> even half, odd half and M^-1 (even half); v *= scale per
-// slot in place; x += merged halves in FP64.
+// FP64 residual -> even half, odd half and M^-1 (even half) (FP32 inner
+// vectors; for FP16 the true residual writes the halves,
+// dslash_residual_eo_split); v *= scale per slot in place; x += merged halves
+// in FP64.
 int eo_split_minv(qcd_hip_field r, int f0, qcd_hip_field ev, qcd_hip_field od, qcd_hip_field tev, double m, double tm,
                   const int *ids, int count, hipStream_t st);
 int eo_scale(qcd_hip_lattice full, qcd_hip_field v, const double *scale, size_t scale_stride, const int *ids, int count,

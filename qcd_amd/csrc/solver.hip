@@ -395,7 +395,10 @@ struct qcd_hip_solver_s {
   qcd_hip_field Rs = nullptr;       // n*B search directions res_j (inner precision)
   qcd_hip_field Ws = nullptr;       // n*B vectors w_j = A res_j (inner precision)
   qcd_hip_field E = nullptr;        // B inner corrections (inner precision)
-  qcd_hip_field r64 = nullptr;      // B FP64 residuals
+  qcd_hip_field r64 = nullptr;      // B FP64 residuals (not allocated with split_res)
+  // FP16-inner even-odd: the true residual is stored straight into the FP16
+  // halves reh, roh and th = M^-1 reh, never as FP64
+  bool split_res = false;
   qcd_hip_field bq = nullptr;       // B FP64 right-hand sides of the solved system
   GcrScal *sc = nullptr;            // B coefficient sets
   double *rn = nullptr;             // B residual norms (device)
@@ -539,11 +542,12 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
     return 0;
   }
   // Mixed precision: only the true residual, the accumulation into x and the
-  // reductions are FP64.  The halves of the residual, the inner source
-  // r^_o and the even correction e_e are formed in the inner precision with
-  // the inner-precision links: their rounding only perturbs the correction
-  // of this outer step, which the next FP64 true residual measures.
-  QH_TRY(eo_split_minv(S->r64, 0, S->reh, S->roh, S->th, m, mu, S->ids, count, st));
+  // reductions are FP64.  The halves of the residual (reh, roh and th =
+  // M^-1 reh; FP16: written by the true residual itself), the inner source
+  // r^_o and the even correction e_e are in the inner precision, with the
+  // inner-precision links: their rounding only perturbs the correction of this
+  // outer step, which the next FP64 true residual measures.
+  if (!S->split_res) QH_TRY(eo_split_minv(S->r64, 0, S->reh, S->roh, S->th, m, mu, S->ids, count, st));
   QH_TRY(eo_launch(EO_SUB, true, S->eoh, 0, 1, S->th, 0, 1, S->gin, S->roh, 0, 1, S->ids, count, m, mu, 1, S->part,
                    st, &grid));
   k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);   // |r^_o| for the scaling
@@ -561,9 +565,14 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
 // r_b = bq_b - A x_{x0+b} for the active slots; residual norms to rn_host
 int true_residual(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double kappa, double mu, hipStream_t st) {
   const int epi = S->use_q ? EPI_RES_Q : EPI_RES_D;
+  const double m = 1.0 / (2.0 * kappa) - 4.0;
   int grid = 0;
-  QH_TRY(dslash_launch_batch(epi, true, S->r64, 0, 1, x, x0, 1, S->g64, S->bq, 0, 1, S->ids, count,
-                             1.0 / (2.0 * kappa) - 4.0, mu, S->part, st, &grid));
+  if (S->split_res)
+    QH_TRY(dslash_residual_eo_split(x, x0, 1, S->g64, S->bq, 0, 1, S->ids, count, m, mu, S->reh, S->roh, S->th,
+                                    S->part, st, &grid));
+  else
+    QH_TRY(dslash_launch_batch(epi, true, S->r64, 0, 1, x, x0, 1, S->g64, S->bq, 0, 1, S->ids, count, m, mu, S->part,
+                               st, &grid));
   k_scal_rnorm<<<count, 1024, 0, st>>>(S->sc, S->part, (unsigned)grid, S->ids, S->rn);
   QH_CHECK(hipGetLastError());
   QH_CHECK(hipMemcpyAsync(S->rn_host, S->rn, sizeof(double) * S->B, hipMemcpyDeviceToHost, st));
@@ -670,7 +679,8 @@ size_t solver_device_bytes(const int L[4], const qcd_hip_solver_params *prm) {
   size_t b = 0;
   b += 2 * field(sti, ip, n * B) + field(sti, ip, B);                  // Rs, Ws, E
   if (prm->eo) b += 5 * field(sti, ip, B);                             // te, reh, roh, th, eoh
-  b += 2 * field(lat.stride, QCD_HIP_F64, B);                          // r64, bq
+  const bool split = prm->eo && ip == QCD_HIP_F16;                     // no r64 (split residual)
+  b += (split ? 1 : 2) * field(lat.stride, QCD_HIP_F64, B);            // r64, bq
   b += sizeof(GcrScal) * B + sizeof(double) * B + sizeof(int) * B;      // sc, rn, ids
   const size_t nb = (size_t)grid_for(V);
   const size_t rowmax = ((size_t)2 * (n + kDotChunk) + 4) * (nb + 64);
@@ -725,7 +735,8 @@ int qcd_hip_solver_create(qcd_hip_solver *out, qcd_hip_lattice lat, const qcd_hi
   if (!rc && prm->eo) rc = qcd_hip_spinor_create(&S->te, ilat, prm->inner_prec, B);
   for (qcd_hip_field *f : {&S->reh, &S->roh, &S->th, &S->eoh})
     if (!rc && prm->eo) rc = qcd_hip_spinor_create(f, ilat, prm->inner_prec, B);
-  if (!rc) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, B);
+  S->split_res = prm->eo && prm->inner_prec == QCD_HIP_F16;
+  if (!rc && !S->split_res) rc = qcd_hip_spinor_create(&S->r64, lat, QCD_HIP_F64, B);
   if (!rc) rc = qcd_hip_spinor_create(&S->bq, lat, QCD_HIP_F64, B);
   if (!rc) rc = dev_alloc((void **)&S->sc, sizeof(GcrScal) * B);
   if (!rc) rc = dev_alloc((void **)&S->rn, sizeof(double) * B);
@@ -823,7 +834,9 @@ int qcd_hip_solver_solve(qcd_hip_solver S, qcd_hip_field x, qcd_hip_field b, dou
   const auto t0 = std::chrono::steady_clock::now();
   // Tune the inner and outer operators before the first timed use.
   if (!S->prm.eo) QH_TRY(dslash_prepare(S->Ws, S->Rs, S->gin));
-  QH_TRY(dslash_prepare(S->r64, x, S->g64));
+  // (scratch output of the tuning launches: r64, or bq, which each batch
+  // fills afterwards)
+  QH_TRY(dslash_prepare(S->r64 ? S->r64 : S->bq, x, S->g64));
   qcd_hip_solver_stats tot;
   std::memset(&tot, 0, sizeof(tot));
   int worst = 0;
