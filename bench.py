@@ -397,7 +397,7 @@ def cpu_solve_leg(n_restart=32):
     el = time.perf_counter() - t
     return {"solves_per_s": 1.0 / el, "seconds": el, "rc": rc, "outer_cycles": outer, "relres": rel,
             "threads": threads, "n_restart": n_restart,
-            "sample": "one rhs of config 2 (16^3x32, 12 origin point sources) solved in full by the oracle"}
+            "sample": "one rhs of the 16^3x32 propagator solve (12 origin point sources) solved in full by the oracle"}
 
 
 def oracle_recheck(kept):
