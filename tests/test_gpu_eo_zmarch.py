@@ -94,3 +94,17 @@ def test_zmarch_eo_solve_converges_like_one_site(gpu, prec):
     finally:
         Q.eo_set_zlen(0)
     assert abs(stats[3]["outer"] - stats[1]["outer"]) <= 1
+
+
+@pytest.mark.parametrize("prec,nrhs", [("f64", 1), ("f64", 13), ("f16", 12)])
+def test_zmarch_schur_bits_equal_one_site_16c32(gpu, prec, nrhs):
+    """16^3x32: 16 column blocks per plane, several z chunks per row and, with
+    13 / 12 rhs, row groups of more than one; z chunks of 3, 4 and 8 and the
+    tuned choice (0) against one site per lane, bit for bit."""
+    Q = gpu
+    L, theta = (32, 16, 16, 16), (np.pi, 0.0, 0.0, 0.0)
+    zl = [1, 3, 4, 8, 0]
+    _, _, _, _, out = _schur(Q, L, theta, prec, nrhs, 970 + nrhs, zl)
+    for z in zl[1:]:
+        for r in range(nrhs):
+            assert np.array_equal(out[z][r], out[1][r]), (prec, nrhs, z, r)
