@@ -221,12 +221,24 @@ __global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, con
   E.store(b, s, o);
 }
 
-// Sum a [nv][nb] partial array (one wave per value, fixed order) into sums.
+// Sum a [nv][nb] partial array (one wave per value, an order fixed by nb)
+// into sums.  Each lane keeps four running sums over partials 256 apart, so
+// its loads do not wait on one chain of additions (rows of 20 000 partials
+// on 48^3x96).
 __device__ void wave_sums(const double *part, int nv, unsigned nb, double *sums) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int v = wid; v < nv; v += nw) {
-    double t = 0.0;
-    for (unsigned i = lane; i < nb; i += 64) t += part[(size_t)v * nb + i];
+    const double *p = part + (size_t)v * nb;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    unsigned i = lane;
+    for (; i + 192 < nb; i += 256) {
+      t0 += p[i];
+      t1 += p[i + 64];
+      t2 += p[i + 128];
+      t3 += p[i + 192];
+    }
+    for (; i < nb; i += 64) t0 += p[i];
+    double t = (t0 + t1) + (t2 + t3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     if (lane == 0) sums[v] = t;
