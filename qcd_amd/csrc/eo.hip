@@ -421,22 +421,6 @@ __global__ void __launch_bounds__(kBlock) k_eo_split_minv(EoArgs e, SpinorSet<QC
   tev.store(b, s >> 1, o);
 }
 
-// v[slot] *= f_slot in place (half vectors, precision P), f_slot = scale[slot]
-template <int P>
-__global__ void __launch_bounds__(kBlock) k_eo_scale(EoArgs e, SpinorSet<P> v, const double *scale,
-                                                     size_t scale_stride, const int *ids) {
-  using R = typename Prec<P>::R;
-  const int b = ids[blockIdx.y];
-  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
-  if (h >= e.Vh) return;
-  const double f = scale[(size_t)b * scale_stride];
-  Cx<R> o[12];
-  v.load(b, h, o);
-#pragma unroll
-  for (int k = 0; k < 12; k++) o[k] = Cx<R>{(R)(f * (double)o[k].re), (R)(f * (double)o[k].im)};
-  v.store(b, h, o);
-}
-
 // x[f0 + slot] += (ev[slot] on even sites, od[slot] on odd sites), the halves
 // in precision P, one lane per site, FP64 accumulation
 template <int P>
@@ -812,14 +796,11 @@ int eo_scale64(qcd_hip_lattice full, qcd_hip_field in, qcd_hip_field out, const 
 
 template <int P>
 static int eo_mixed_launch(int what, qcd_hip_lattice full, qcd_hip_field a, int f0, qcd_hip_field ev, qcd_hip_field od,
-                           qcd_hip_field tev, double m, double tm, const double *scale, size_t scale_stride,
-                           const int *ids, int count, hipStream_t st) {
+                           qcd_hip_field tev, double m, double tm, const int *ids, int count, hipStream_t st) {
   const EoArgs e = make_eo_args(full, m, tm, 0);
   if (what == 0) {
     k_eo_split_minv<P><<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(
         e, SpinorSet<QCD_HIP_F64>::of(a), f0, SpinorSet<P>::of(ev), SpinorSet<P>::of(od), SpinorSet<P>::of(tev), ids);
-  } else if (what == 1) {
-    k_eo_scale<P><<<dim3(grid_for(e.Vh), count), kBlock, 0, st>>>(e, SpinorSet<P>::of(ev), scale, scale_stride, ids);
   } else {
     k_eo_accumulate_p<P><<<dim3(grid_for(e.d.V), count), kBlock, 0, st>>>(e, SpinorSet<QCD_HIP_F64>::of(a), f0,
                                                                          SpinorSet<P>::of(ev), SpinorSet<P>::of(od),
@@ -830,13 +811,12 @@ static int eo_mixed_launch(int what, qcd_hip_lattice full, qcd_hip_field a, int 
 }
 
 static int eo_mixed(int what, qcd_hip_lattice full, qcd_hip_field a, int f0, qcd_hip_field ev, qcd_hip_field od,
-                    qcd_hip_field tev, double m, double tm, const double *scale, size_t scale_stride, const int *ids,
-                    int count, hipStream_t st) {
+                    qcd_hip_field tev, double m, double tm, const int *ids, int count, hipStream_t st) {
   switch (ev->prec) {
     case QCD_HIP_F32:
-      return eo_mixed_launch<QCD_HIP_F32>(what, full, a, f0, ev, od, tev, m, tm, scale, scale_stride, ids, count, st);
+      return eo_mixed_launch<QCD_HIP_F32>(what, full, a, f0, ev, od, tev, m, tm, ids, count, st);
     case QCD_HIP_F16:
-      return eo_mixed_launch<QCD_HIP_F16>(what, full, a, f0, ev, od, tev, m, tm, scale, scale_stride, ids, count, st);
+      return eo_mixed_launch<QCD_HIP_F16>(what, full, a, f0, ev, od, tev, m, tm, ids, count, st);
     default:
       set_error("eo: the inner-precision outer step needs FP32 or FP16 half vectors");
       return 1;
@@ -845,17 +825,12 @@ static int eo_mixed(int what, qcd_hip_lattice full, qcd_hip_field a, int f0, qcd
 
 int eo_split_minv(qcd_hip_field r, int f0, qcd_hip_field ev, qcd_hip_field od, qcd_hip_field tev, double m, double tm,
                   const int *ids, int count, hipStream_t st) {
-  return eo_mixed(0, r->lat, r, f0, ev, od, tev, m, tm, nullptr, 0, ids, count, st);
-}
-
-int eo_scale(qcd_hip_lattice full, qcd_hip_field v, const double *scale, size_t scale_stride, const int *ids, int count,
-             hipStream_t st) {
-  return eo_mixed(1, full, nullptr, 0, v, nullptr, nullptr, 0.0, 0.0, scale, scale_stride, ids, count, st);
+  return eo_mixed(0, r->lat, r, f0, ev, od, tev, m, tm, ids, count, st);
 }
 
 int eo_accumulate_p(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
                     hipStream_t st) {
-  return eo_mixed(2, x->lat, x, f0, ev, od, nullptr, 0.0, 0.0, nullptr, 0, ids, count, st);
+  return eo_mixed(2, x->lat, x, f0, ev, od, nullptr, 0.0, 0.0, ids, count, st);
 }
 
 }  // namespace qcdhip

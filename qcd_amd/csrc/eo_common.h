@@ -96,12 +96,9 @@ int eo_launch(int epi, bool red, qcd_hip_field out, int ro, int out_rs, qcd_hip_
 // The outer step of a mixed even-odd solve in the inner precision (eo.hip):
 // FP64 residual -> even half, odd half and M^-1 (even half) (FP32 inner
 // vectors; for FP16 the true residual writes the halves,
-// dslash_residual_eo_split); v *= scale per slot in place; x += merged halves
-// in FP64.
+// dslash_residual_eo_split); x += merged halves in FP64.
 int eo_split_minv(qcd_hip_field r, int f0, qcd_hip_field ev, qcd_hip_field od, qcd_hip_field tev, double m, double tm,
                   const int *ids, int count, hipStream_t st);
-int eo_scale(qcd_hip_lattice full, qcd_hip_field v, const double *scale, size_t scale_stride, const int *ids, int count,
-             hipStream_t st);
 int eo_accumulate_p(qcd_hip_field x, int f0, qcd_hip_field ev, qcd_hip_field od, const int *ids, int count,
                     hipStream_t st);
 // the same hops of an FP16 batch on the matrix cores (mma.hip): EO_MINV and

@@ -190,14 +190,16 @@ __global__ void __launch_bounds__(kBlock) k_res_update(SpinorSet<P> Rs, SpinorSe
   Rs.store(b * n + k + 1, s, r);
 }
 
-// e_b = sum_j beta_j res_j
+// e_b = sum_j beta_j res_j, times |r_b| (the norm the inner source was
+// divided by) when scaled
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_solution(SpinorSet<P> Rs, int n, const GcrScal *sc, SpinorSet<P> E,
-                                                     const int *ids, unsigned V) {
+                                                     const int *ids, unsigned V, int scaled) {
   using R = typename Prec<P>::R;
   const int b = ids[blockIdx.y];
   __shared__ double2 bt[kN];
-  for (int j = threadIdx.x; j < n; j += kBlock) bt[j] = sc[b].beta[j];
+  const double f = scaled ? sc[b].rnorm : 1.0;
+  for (int j = threadIdx.x; j < n; j += kBlock) bt[j] = make_double2(sc[b].beta[j].x * f, sc[b].beta[j].y * f);
   __syncthreads();
   const unsigned s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= V) return;
@@ -473,7 +475,10 @@ int gcr_cycle(qcd_hip_solver_s *S, int count, double kappa, double mu, hipStream
     if (k + 1 < n) k_res_update<P><<<g2, kBlock, 0, st>>>(Rs, Ws, n, k, S->sc, S->ids, V);
   }
   k_scal_beta<<<count, kN, 0, st>>>(S->sc, n, S->prm.keep_q1, S->ids);
-  k_solution<P><<<g2, kBlock, 0, st>>>(Rs, n, S->sc, E, S->ids, V);
+  // mixed even-odd solves take the correction scaled by |r^_o| (the outer
+  // step adds it as it is); the others scale it where they add it
+  const int scaled = S->prm.eo && S->prm.inner_prec != QCD_HIP_F64;
+  k_solution<P><<<g2, kBlock, 0, st>>>(Rs, n, S->sc, E, S->ids, V, scaled);
   QH_CHECK(hipGetLastError());
   return 0;
 }
@@ -567,7 +572,6 @@ int outer_step(qcd_hip_solver_s *S, qcd_hip_field x, int x0, int count, double k
                                              S->prm.n_restart, S->ids, V);
   QH_CHECK(hipGetLastError());
   QH_TRY(run_cycle<P>(S, count, kappa, mu, st));
-  QH_TRY(eo_scale(S->lat, S->E, &S->sc[0].rnorm, sstride, S->ids, count, st));   // e_o = |r^_o| E
   QH_TRY(eo_launch(EO_SUB_MINV, false, S->th, 0, 1, S->E, 0, 1, S->gin, S->reh, 0, 1, S->ids, count, m, mu, 0,
                    nullptr, st, nullptr));
   QH_TRY(eo_accumulate_p(x, x0, S->th, S->E, S->ids, count, st));
