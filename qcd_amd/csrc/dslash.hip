@@ -789,12 +789,15 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
       if (ms < best.ms) best = TuneVal{map, occ, ms};
     }
   }
-  // Batches (FP32/FP16 fields with several rhs): the best one-rhs traversal
-  // against two rhs per lane and the matrix cores, timed over the first
-  // min(nrhs, 48) rhs (at 24^3x48 the matrix cores trail at 16 rhs and lead
-  // at 48, so a choice made on fewer rhs than the batch can be wrong).
+  // Batches (fields with several rhs): the best one-rhs traversal against
+  // other traversals, timed over the first min(nrhs, 48) rhs (at 24^3x48 the
+  // matrix cores trail at 16 rhs and lead at 48, so a choice made on fewer rhs
+  // than the batch can be wrong).  FP32/FP16: two rhs per lane and the matrix
+  // cores.  FP64 (one rhs per lane): other z-march lengths and row tiles; over
+  // 48 rhs at 24^3x48 the z-march of 8 steps takes 5.8 ms against 6.8 for the
+  // one-rhs choice (LDS carry, 12 steps).
   const int cnt = std::min(std::min(in->nrhs, out->nrhs), 48);
-  if (cnt >= 2 && in->prec != QCD_HIP_F64) {
+  if (cnt >= 2) {
     const Batch bt{nullptr, cnt, 1, 1, 1};
     float bbest = 1e30f;
     auto time_batch = [&](int map, int occ, float *ms) -> int {
@@ -811,6 +814,24 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
     };
     QH_TRY(time_batch(best.map, best.occ, &bbest));
     const float one = bbest;
+    if (in->prec == QCD_HIP_F64) {
+      for (int map : {MAP_ZMARCH | (4 << 8), MAP_ZMARCH | (6 << 8), MAP_ZMARCH | (8 << 8), MAP_ZMARCH | (12 << 8),
+                      MAP_ZMARCH_LDS | (8 << 8), MAP_ZTILE | (4 << 8), MAP_ZTILE | (8 << 8)})
+        for (int occ : {0, 16}) {
+          if (map == best.map && occ == best.occ) continue;
+          float ms = 0.f;
+          QH_TRY(time_batch(map, occ, &ms));
+          if (getenv("QCD_HIP_TUNE_VERBOSE"))
+            fprintf(stderr, "tune L=%dx%dx%dx%d ps=%d pg=%d %d rhs map=%d zlen=%d occ=%d: %.2f us (one-rhs %.2f)\n",
+                    key.L[0], key.L[1], key.L[2], key.L[3], key.ps, key.pg, cnt, map & 0xff, map >> 8, occ, ms * 1e3,
+                    one * 1e3);
+          if (ms < bbest) {
+            bbest = ms;
+            best.bmap = map;
+            best.bocc = occ;
+          }
+        }
+    }
     if (in->prec == QCD_HIP_F16 && g->prec == QCD_HIP_F16) {
       float ms = 0.f;
       QH_TRY(time_batch(MAP_MMA, 0, &ms));
@@ -823,19 +844,21 @@ int dslash_tune(qcd_hip_field out, qcd_hip_field in, qcd_hip_field g, int force)
         best.bocc = 0;
       }
     }
-    for (int z : {2, 3, 4, 6, 8})
-      for (int occ : {0, 16}) {
-        float ms = 0.f;
-        QH_TRY(time_batch(MAP_ZMARCH2 | (z << 8), occ, &ms));
-        if (getenv("QCD_HIP_TUNE_VERBOSE"))
-          fprintf(stderr, "tune L=%dx%dx%dx%d ps=%d pg=%d %d rhs two-rhs lanes zlen=%d occ=%d: %.2f us (one-rhs %.2f)\n",
-                  key.L[0], key.L[1], key.L[2], key.L[3], key.ps, key.pg, cnt, z, occ, ms * 1e3, one * 1e3);
-        if (ms < bbest) {
-          bbest = ms;
-          best.bmap = MAP_ZMARCH2 | (z << 8);
-          best.bocc = occ;
+    if (in->prec != QCD_HIP_F64)
+      for (int z : {2, 3, 4, 6, 8})
+        for (int occ : {0, 16}) {
+          float ms = 0.f;
+          QH_TRY(time_batch(MAP_ZMARCH2 | (z << 8), occ, &ms));
+          if (getenv("QCD_HIP_TUNE_VERBOSE"))
+            fprintf(stderr,
+                    "tune L=%dx%dx%dx%d ps=%d pg=%d %d rhs two-rhs lanes zlen=%d occ=%d: %.2f us (one-rhs %.2f)\n",
+                    key.L[0], key.L[1], key.L[2], key.L[3], key.ps, key.pg, cnt, z, occ, ms * 1e3, one * 1e3);
+          if (ms < bbest) {
+            bbest = ms;
+            best.bmap = MAP_ZMARCH2 | (z << 8);
+            best.bocc = occ;
+          }
         }
-      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -901,7 +924,11 @@ int dslash_launch_batch(int epi, bool red, qcd_hip_field out, int ro, int out_rs
   if (count < 1) return 0;
   TuneVal t;
   const bool have = tune_get(key_of(in, g), &t);
-  const bool two = have && count >= 2 && t.bmap != 0;
+  // the batch traversal applies where it was timed: FP64 batches were tuned on
+  // D without reduction (the residual epilogues need more registers and keep
+  // the one-rhs choice); FP32/FP16 residuals fall back inside launch_cfg
+  const bool timed = in->prec != QCD_HIP_F64 || (!red && (epi == EPI_D || epi == EPI_Q));
+  const bool two = have && count >= 2 && t.bmap != 0 && timed;
   const int map = two ? t.bmap : have ? t.map : kDefaultMap, occ = two ? t.bocc : have ? t.occ : 0;
   const Batch bt{ids, count, in_rs, out_rs, b_rs};
   return launch_cfg(epi, red, map, occ, out, ro, in, ri, g, b, rb, m, tm, partials, stream, grid_out, bt);
@@ -924,8 +951,10 @@ int dslash_residual_eo_split(qcd_hip_field x, int x0, int x_rs, qcd_hip_field g,
   }
   TuneVal t;
   const bool have = tune_get(key_of(x, g), &t);
-  const bool two = have && count >= 2 && t.bmap != 0;
-  const int map = two ? t.bmap : have ? t.map : kDefaultMap, occ = two ? t.bocc : have ? t.occ : 0;
+  // the one-rhs choice: the FP64 batch choice was timed on D, and with this
+  // epilogue's register load it measured slower (row tiles of 8 steps: 6.7
+  // against 6.3 ms over 48 rhs of 24^3x48)
+  const int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
   const Batch bt{ids, count, x_rs, 0, b_rs};
   const double a4 = m + 4.0, den = a4 * a4 + tm * tm;   // M^-1 as make_eo_args forms it
   const EoSplit<QCD_HIP_F16> sp{SpinorOut<QCD_HIP_F16>::of(ev, 0), SpinorOut<QCD_HIP_F16>::of(od, 0),

@@ -217,6 +217,44 @@ def test_two_rhs_lanes_match_one_rhs_lanes(gpu):
                 assert np.array_equal(a, b), prec
 
 
+def test_fp64_batch_traversals_match_one_rhs(gpu):
+    """FP64 batches: the traversals the tuner times for batches (z-march of 4,
+    6, 8 and 12 steps, LDS carry of 8, row tiles of 4 and 8; with and without
+    the scheduling fence) and the tuned batch choice give every rhs of a
+    5-rhs batch the bits of a one-rhs launch, on a lattice with ragged z
+    chunks and row tiles; rhs 0 equals the oracle."""
+    Q = gpu
+    L = (16, 6, 4, 20)
+    m = 1 / (2 * 0.13) - 4
+    u, _ = _fields(L, 0.3, 181, 182)
+    lat = Q.Lattice(L)
+    g = Q.Gauge(lat).upload(u)
+    psis = [Q.spinor_gaussian(L, 190 + r) for r in range(5)]
+    x = Q.Spinor(lat, nrhs=5)
+    for r, v in enumerate(psis):
+        x.upload(v, rhs=r)
+    y = Q.Spinor(lat, nrhs=5)
+    x1, y1 = Q.Spinor(lat), Q.Spinor(lat)
+    for op in (Q.OP_Q, Q.OP_D):
+        one = []
+        for v in psis:
+            x1.upload(v)
+            Q.dslash(y1, x1, g, m, 0.02, op)
+            one.append(y1.download())
+        ref = (O.qtm if op == Q.OP_Q else O.dslash)(O.Geometry(L), psis[0], u, m, 0.02)
+        assert rel_l2(one[0], ref) <= 1e-13
+        for map_ in (2 | (4 << 8), 2 | (6 << 8), 2 | (8 << 8), 2 | (12 << 8), 4 | (8 << 8), 3 | (4 << 8), 3 | (8 << 8)):
+            for occ in (0, 16):
+                Q.dslash_set_config(x, g, map_, occ)
+                Q.dslash(y, x, g, m, 0.02, op)
+                for r in range(5):
+                    assert np.array_equal(y.download(r), one[r]), (op, map_ & 0xff, map_ >> 8, occ, r)
+        Q.dslash_tune(y, x, g)
+        Q.dslash(y, x, g, m, 0.02, op)
+        for r in range(5):
+            assert np.array_equal(y.download(r), one[r]), (op, "tuned", r)
+
+
 @pytest.mark.parametrize("L", [(16, 8, 8, 8), (16, 6, 4, 20)])
 def test_every_tuner_candidate(gpu, L):
     """Every traversal the autotuner can pick (qcd_hip_dslash_candidates) with
