@@ -60,7 +60,7 @@ struct SpinorSet {
       for (int k = 0; k < 12; k++) mx = fmaxf(mx, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
       if (!(mx >= FLT_MIN)) mx = R(0);
       const R inv = mx > R(0) ? R(1) / mx : R(0);
-      norm[(size_t)j * nstride + s] = mx;
+      __builtin_nontemporal_store((float)mx, &norm[(size_t)j * nstride + s]);
 #pragma unroll
       for (int g = 0; g < 3; g++) {
         u32x4 q;
@@ -70,7 +70,7 @@ struct SpinorSet {
           const Cx<R> c = to_cx<R>(as_half2(q[i]));
           v[4 * g + i] = Cx<R>{c.re * mx, c.im * mx};
         }
-        *(u32x4 *)((char *)b + g * 4 * kTile * 4 + off) = q;
+        __builtin_nontemporal_store(q, (u32x4 *)((char *)b + g * 4 * kTile * 4 + off));
       }
     } else {
       store(j, s, v);
@@ -87,23 +87,28 @@ struct SpinorSet {
       // stored as zero: 1/mx would overflow and 0 * inf give NaN
       if (!(mx >= FLT_MIN)) mx = R(0);
       const R inv = mx > R(0) ? R(1) / mx : R(0);
-      norm[(size_t)j * nstride + s] = mx;
+      __builtin_nontemporal_store((float)mx, &norm[(size_t)j * nstride + s]);
 #pragma unroll
       for (int g = 0; g < 3; g++) {
         u32x4 q;
 #pragma unroll
         for (int i = 0; i < 4; i++)
           q[i] = as_u32(from_cx<T>(Cx<R>{v[4 * g + i].re * inv, v[4 * g + i].im * inv}));
-        *(u32x4 *)((char *)b + g * 4 * kTile * 4 + off) = q;
+        __builtin_nontemporal_store(q, (u32x4 *)((char *)b + g * 4 * kTile * 4 + off));
       }
     } else if constexpr (P == QCD_HIP_F32) {
+      typedef float f4v_ __attribute__((ext_vector_type(4)));
 #pragma unroll
       for (int g = 0; g < 6; g++)
-        *(float4 *)((char *)b + g * 2 * kTile * 8 + off) =
-            make_float4((float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im);
+        __builtin_nontemporal_store(
+            f4v_{(float)v[2 * g].re, (float)v[2 * g].im, (float)v[2 * g + 1].re, (float)v[2 * g + 1].im},
+            (f4v_ *)((char *)b + g * 2 * kTile * 8 + off));
     } else {
+      typedef double d2v_ __attribute__((ext_vector_type(2)));
 #pragma unroll
-      for (int k = 0; k < 12; k++) st_plane(b + k * kTile, off, from_cx<T>(v[k]));
+      for (int k = 0; k < 12; k++)
+        __builtin_nontemporal_store(d2v_{(double)v[k].re, (double)v[k].im},
+                                    (d2v_ *)((char *)(b + k * kTile) + off));
     }
   }
 };

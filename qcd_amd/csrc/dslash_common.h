@@ -142,29 +142,39 @@ struct SpinorIn {
   static __device__ __forceinline__ unsigned offset(unsigned s) {
     return ((s / kTile) * (12u * kTile) + (s % kTile) * (unsigned)kG) * (unsigned)sizeof(T);
   }
-  // stored values of site s (and its FP16 norm)
+  // stored values of site s (and its FP16 norm); NT: nontemporal
+  // (streaming) loads, for values read once
+  template <bool NT = false>
   __device__ __forceinline__ void load_raw(unsigned s, T r[12], float &n) const {
     const unsigned off = offset(s);
     if constexpr (P == QCD_HIP_F16) {
 #pragma unroll
       for (int g = 0; g < 3; g++) {
-        const u32x4 q = *(const u32x4 *)((const char *)p[0] + g * 4 * kTile * 4 + off);
+        const u32x4 *a = (const u32x4 *)((const char *)p[0] + g * 4 * kTile * 4 + off);
+        u32x4 q;
+        if constexpr (NT) q = __builtin_nontemporal_load(a);
+        else q = *a;
 #pragma unroll
         for (int i = 0; i < 4; i++) r[4 * g + i] = as_half2(q[i]);
       }
-      n = ld_plane(norm, s * 4u);
+      if constexpr (NT) n = __builtin_nontemporal_load((const float *)((const char *)norm + s * 4u));
+      else n = ld_plane(norm, s * 4u);
     } else if constexpr (P == QCD_HIP_F32) {
+      typedef float f4v_ __attribute__((ext_vector_type(4)));
 #pragma unroll
       for (int g = 0; g < 6; g++) {
         // groups 4 and 5 start at the second base (element 8 * 64), so
         // every immediate offset stays below 4096 bytes
-        const float4 q = *(const float4 *)((const char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off);
+        const f4v_ *a = (const f4v_ *)((const char *)p[g / 4] + (g % 4) * 2 * kTile * 8 + off);
+        f4v_ q;
+        if constexpr (NT) q = __builtin_nontemporal_load(a);
+        else q = *a;
         r[2 * g] = make_float2(q.x, q.y);
         r[2 * g + 1] = make_float2(q.z, q.w);
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) r[k] = ld_plane(comp(k), off);
+      for (int k = 0; k < 12; k++) r[k] = NT ? ld_plane_nt(comp(k), off) : ld_plane(comp(k), off);
     }
   }
   static __device__ __forceinline__ void convert(const T r[12], float n, Cx<R> v[12]) {
@@ -178,10 +188,11 @@ struct SpinorIn {
       }
     }
   }
+  template <bool NT = false>
   __device__ __forceinline__ void load(unsigned s, Cx<R> v[12]) const {
     T r[12];
     float n = 0.f;
-    load_raw(s, r, n);
+    load_raw<NT>(s, r, n);
     convert(r, n, v);
   }
 };
@@ -741,8 +752,9 @@ __device__ __forceinline__ void apply_epilogue(Cx<R> o[12], const SpinorIn<PS> &
     for (int k = 6; k < 12; k++) o[k] = ipow<3>(o[k]);
   }
   if constexpr (EPI == EPI_RES_D || EPI == EPI_RES_Q || epi_eo_split<EPI>()) {
+    // b is read once: a streaming load leaves L2 to the neighbour planes
     Cx<R> bv[12];
-    b.load(s, bv);
+    b.template load<true>(s, bv);
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = Cx<R>{bv[k].re - o[k].re, bv[k].im - o[k].im};
   }

@@ -77,8 +77,9 @@ __device__ __forceinline__ void eo_epilogue(Cx<R> o[12], const Cx<R> acc[4][3], 
       for (int c = 0; c < 3; c++) o[3 * q + c] = Cx<R>{R(-0.5) * acc[q][c].re, R(-0.5) * acc[q][c].im};
     eo_mass<true>(o, e);
   } else {
+    // b is read once: a streaming load leaves L2 to the neighbour planes
     Cx<R> bv[12];
-    b.load(h, bv);
+    b.template load<true>(h, bv);
     if constexpr (EPI == EO_SCHUR) eo_mass<false>(bv, e);
 #pragma unroll
     for (int q = 0; q < 4; q++)
