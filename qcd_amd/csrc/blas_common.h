@@ -27,10 +27,10 @@ struct SpinorSet {
     const T *b = base + (size_t)j * vstride;
     const unsigned off = SpinorIn<P>::offset(s);
     if constexpr (P == QCD_HIP_F16) {
-      const R n = norm[(size_t)j * nstride + s];
+      const R n = __builtin_nontemporal_load(&norm[(size_t)j * nstride + s]);
 #pragma unroll
       for (int g = 0; g < 3; g++) {
-        const u32x4 q = *(const u32x4 *)((const char *)b + g * 4 * kTile * 4 + off);
+        const u32x4 q = __builtin_nontemporal_load((const u32x4 *)((const char *)b + g * 4 * kTile * 4 + off));
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const Cx<R> c = to_cx<R>(as_half2(q[i]));
@@ -40,13 +40,14 @@ struct SpinorSet {
     } else if constexpr (P == QCD_HIP_F32) {
 #pragma unroll
       for (int g = 0; g < 6; g++) {
-        const float4 q = *(const float4 *)((const char *)b + g * 2 * kTile * 8 + off);
+        typedef float f4v_ __attribute__((ext_vector_type(4)));
+        const f4v_ q = __builtin_nontemporal_load((const f4v_ *)((const char *)b + g * 2 * kTile * 8 + off));
         v[2 * g] = Cx<R>{q.x, q.y};
         v[2 * g + 1] = Cx<R>{q.z, q.w};
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane(b + k * kTile, off));
+      for (int k = 0; k < 12; k++) v[k] = to_cx<R>(ld_plane_nt(b + k * kTile, off));
     }
   }
   // store v and leave in v the stored values (FP16: rounded to the stored
