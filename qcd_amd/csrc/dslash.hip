@@ -160,7 +160,8 @@ k_dslash_zm(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
         local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
         // U_z(s) is read by this lane only (the next step carries the hop it
         // feeds): a streaming load leaves L2 to the spinor planes
-        U.template load_raw<true>(3, s, u);
+        // residual epilogues run over batch rows whose groups share links: a normal load
+        U.template load_raw<EPI == EPI_D || EPI == EPI_Q>(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
         hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
@@ -421,7 +422,8 @@ k_dslash_zt(DslashArgs a, SpinorIn<PS> in_all, GaugeIn<PG> U, SpinorOut<PS> out_
         local_terms<PS>(acc, rc, nc, cz0, cz1, mp4, tm);
         // U_z(s) is read by this lane only (the next step carries the hop it
         // feeds): a streaming load leaves L2 to the spinor planes
-        U.template load_raw<true>(3, s, u);
+        // residual epilogues run over batch rows whose groups share links: a normal load
+        U.template load_raw<EPI == EPI_D || EPI == EPI_Q>(3, s, u);
         // the next site's backward z hop: U_z(s)^+ (1 + g_z) psi(s)
         hop_half_raw<3, +1, true, PS, PG>(cz0, cz1, rc, nc, u, phm);
         const unsigned sf = z + 1 == a.L[3] ? col : s + P3;
