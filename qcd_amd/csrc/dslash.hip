@@ -13,6 +13,9 @@
 //
 // Epilogues: D, Q = i g5 D (lib/src/qcd_wilson.c:805-812), and the residuals
 // b - D x, b - Q x used by the solver, optionally with |out|^2 block partials.
+#include <array>
+#include <vector>
+
 #include "dslash_common.h"
 
 namespace qcdhip {
@@ -953,14 +956,76 @@ int dslash_residual_eo_split(qcd_hip_field x, int x0, int x_rs, qcd_hip_field g,
   }
   TuneVal t;
   const bool have = tune_get(key_of(x, g), &t);
-  // the one-rhs choice: the FP64 batch choice was timed on D, and with this
-  // epilogue's register load it measured slower (row tiles of 8 steps: 6.7
-  // against 6.3 ms over 48 rhs of 24^3x48)
-  const int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
+  // the one-rhs choice of D to start with (the FP64 batch choice was timed on
+  // D and measured slower here: row tiles of 8 steps, 6.7 against 6.3 ms over
+  // 48 rhs of 24^3x48)
+  int map = have ? t.map : kDefaultMap, occ = have ? t.occ : 0;
   const Batch bt{ids, count, x_rs, 0, b_rs};
   const double a4 = m + 4.0, den = a4 * a4 + tm * tm;   // M^-1 as make_eo_args forms it
   const EoSplit<QCD_HIP_F16> sp{SpinorOut<QCD_HIP_F16>::of(ev, 0), SpinorOut<QCD_HIP_F16>::of(od, 0),
                                 SpinorOut<QCD_HIP_F16>::of(tev, 0), a4 / den, -tm / den};
+  if (count >= 2 && autotune_enabled()) {
+    // This epilogue's register load differs from D's, and so does its best
+    // traversal: the first batched call per lattice outside graph capture
+    // times a few candidates on the call itself (each launch rewrites only the
+    // outputs and partials; every candidate's grid is within
+    // dslash_grid_bound).  24^3x48, 48 rhs: LDS-carry z-march of 12 steps
+    // 5.8 ms against 6.4 for row tiles of 8, two choices the one-rhs D tuner
+    // finds within 1% of each other.
+    std::lock_guard<std::recursive_mutex> lock(state_mutex());
+    static std::vector<std::pair<std::array<int, 4>, std::pair<int, int>>> tuned;
+    const std::array<int, 4> key{(int)x->lat->L[0], (int)x->lat->L[1], (int)x->lat->L[2], (int)x->lat->L[3]};
+    bool found = false;
+    for (const auto &e : tuned)
+      if (e.first == key) {
+        map = e.second.first;
+        occ = e.second.second;
+        found = true;
+      }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    QH_CHECK(hipStreamIsCapturing(stream, &cs));
+    if (!found && cs == hipStreamCaptureStatusNone) {
+      hipEvent_t e0, e1;
+      QH_CHECK(hipEventCreate(&e0));
+      QH_CHECK(hipEventCreate(&e1));
+      float best = 1e30f;
+      int bmap = map, bocc = occ, rc = 0;
+      const int cands[][2] = {{map, occ},
+                              {MAP_ZMARCH_LDS | (12 << 8), 16}, {MAP_ZMARCH_LDS | (8 << 8), 16},
+                              {MAP_ZMARCH | (8 << 8), 16},      {MAP_ZMARCH | (4 << 8), 16},
+                              {MAP_ZTILE | (8 << 8), 16},       {MAP_ZTILE | (4 << 8), 16}};
+      for (const auto &c : cands) {
+        rc = launch_cfg(EPI_RES_D_EO16, true, c[0], c[1], x, 0, x, x0, g, b, rb, m, tm, partials, stream, nullptr,
+                        bt, &sp);
+        if (!rc && hipEventRecord(e0, stream) != hipSuccess) rc = 2;
+        for (int r = 0; r < 2 && !rc; r++)
+          rc = launch_cfg(EPI_RES_D_EO16, true, c[0], c[1], x, 0, x, x0, g, b, rb, m, tm, partials, stream, nullptr,
+                          bt, &sp);
+        float ms = 0.f;
+        if (!rc && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = 2;
+        if (rc) break;
+        if (getenv("QCD_HIP_TUNE_VERBOSE"))
+          fprintf(stderr, "tune split residual L=%dx%dx%dx%d %d rhs map=%d zlen=%d occ=%d: %.2f us\n", key[0], key[1],
+                  key[2], key[3], count, c[0] & 0xff, c[0] >> 8, c[1], ms * 500.f);
+        if (ms < best) {
+          best = ms;
+          bmap = c[0];
+          bocc = c[1];
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (rc) {
+        if (rc == 2) set_error("dslash: timing the split residual failed: %s", hipGetErrorString(hipGetLastError()));
+        return rc;
+      }
+      map = bmap;
+      occ = bocc;
+      tuned.push_back({key, {map, occ}});
+    }
+  }
   return launch_cfg(EPI_RES_D_EO16, true, map, occ, x, 0, x, x0, g, b, rb, m, tm, partials, stream, grid_out, bt, &sp);
 }
 
